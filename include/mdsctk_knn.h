@@ -1,0 +1,144 @@
+/*
+ * mdsctk_knn.h - C ABI of the MI355X knn_rms engine.
+ *
+ * The reference (MDSCTK 1.2.5) has no library interface for this path: the
+ * knn_rms executable (knn_rms.cpp) is the contract.  Inside it, two pieces
+ * are replaced by this library:
+ *   - double distance(int natoms, real w[], coord_array ref,
+ *                     coord_array fit, bool dofit)       knn_rms.cpp:38-41
+ *     called for every (fit, ref) pair from the OpenMP loops at
+ *     knn_rms.cpp:231-243 and 256-279, and
+ *   - permutation<double>::sort(k1)                       mdsctk.h:177-199
+ *     with the "drop rank 0, write ranks 1..k" row writes at
+ *     knn_rms.cpp:246-255 and 281-291.
+ * The frame loading and centring (knn_rms.cpp:149-206) maps to
+ * mdsctk_knn_set_reference / the query upload; file formats are in
+ * mdsctk_io.h.
+ *
+ * Conventions
+ *   - Coordinates are float32 nm, frame-major [n][natoms][3], uncentred; the
+ *     library centres them with the mass weights (knn_rms.cpp:189,199).
+ *   - Distances are returned in Angstrom (knn_rms.cpp:40 multiplies by 10),
+ *     indices are 0-based reference frame numbers, rows in query order.
+ *   - Return value: >= 0 on success, a negative MDSCTK_ERR_* code on
+ *     failure; mdsctk_knn_last_error() describes the failure.  The library
+ *     never calls exit().
+ *   - A context is used from one host thread at a time.  It owns one HIP
+ *     stream and all device memory it allocates.  Several contexts may exist
+ *     at once (e.g. one per resident reference set).
+ */
+#ifndef MDSCTK_KNN_H
+#define MDSCTK_KNN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+#define MDSCTK_OK 0
+#define MDSCTK_ERR_ARG (-1)      /* invalid argument */
+#define MDSCTK_ERR_HIP (-2)      /* HIP runtime failure */
+#define MDSCTK_ERR_NOMEM (-3)    /* device allocation failed or HBM budget exceeded */
+#define MDSCTK_ERR_IO (-4)       /* file could not be opened / read / written */
+#define MDSCTK_ERR_FORMAT (-5)   /* malformed input file */
+#define MDSCTK_ERR_NATOMS (-6)   /* atom count mismatch (knn_rms.cpp:158-178 exits 4) */
+#define MDSCTK_ERR_STATE (-7)    /* no reference set, or request unknown */
+#define MDSCTK_ERR_PENDING (-8)  /* request still running (mdsctk_knn_poll) */
+
+/* Precision of the cross-covariance contraction.  Every form re-scores its
+ * candidate shortlist in FP64, so the returned distances are FP64-exact;
+ * the form only changes how candidates are found (and the speed). */
+#define MDSCTK_PREC_FP32 0        /* FP32 MFMA (v_mfma_f32_32x32x2_f32) */
+#define MDSCTK_PREC_SPLIT_BF16 1  /* hi/lo BF16, 3 MFMA products, FP32 accumulate */
+#define MDSCTK_PREC_SPLIT_FP16 2  /* hi/lo FP16, 3 MFMA products, FP32 accumulate */
+#define MDSCTK_PREC_EXACT_FP64 3  /* FP64 for every pair (no MFMA); slow, for checks */
+
+/* Query flags. */
+#define MDSCTK_KNN_NOFIT 1u       /* --nofit: centre only, no rotation (knn_rms.cpp:39) */
+#define MDSCTK_KNN_FULL_ROWS 2u   /* --sort false: full unsorted rows, identity indices */
+#define MDSCTK_KNN_KEEP_RANK0 4u  /* keep the nearest neighbour instead of dropping it */
+
+/* Options for mdsctk_knn_set_option. */
+#define MDSCTK_OPT_HBM_BUDGET 1   /* bytes of device memory the context may hold (0 = no limit) */
+#define MDSCTK_OPT_EPS_SCALE 2    /* multiplier on the candidate window (default 1) */
+#define MDSCTK_OPT_REF_SPLITS 3   /* force the number of reference phases (0 = automatic) */
+
+typedef struct mdsctk_knn_ctx mdsctk_knn_ctx;
+
+/* Timings and counters of the last completed query (device times from HIP
+ * events on the context's stream). */
+typedef struct mdsctk_knn_stats {
+  double ms_prep;          /* centring/operand pass over the query frames */
+  double ms_contract;      /* fused contraction + QCP + top-k list kernel */
+  double ms_merge;         /* candidate selection + FP64 re-score + output */
+  double ms_fallback;      /* exact rows for queries whose shortlist overflowed */
+  double ms_total;         /* device time of the whole query (incl. copies) */
+  int64_t n_pairs;         /* n_fit * n_ref */
+  int64_t n_candidates;    /* candidates re-scored in FP64 */
+  int64_t n_fallback_rows; /* rows recomputed exactly */
+  int32_t ref_splits;      /* reference phases used by the contraction */
+  int32_t list_capacity;   /* per-list candidate capacity */
+  double eps_rel;          /* candidate window relative to (G_q + max G_ref) */
+} mdsctk_knn_stats;
+
+/* Create a context on HIP device `device` with the given MDSCTK_PREC_*.
+ * Replaces nothing in the reference (the reference has no device). */
+int mdsctk_knn_create(mdsctk_knn_ctx** ctx, int device, int precision);
+
+/* Release all device memory and the stream. */
+int mdsctk_knn_destroy(mdsctk_knn_ctx* ctx);
+
+/* Text of the last error on this context (never NULL). */
+const char* mdsctk_knn_last_error(const mdsctk_knn_ctx* ctx);
+
+int mdsctk_knn_set_option(mdsctk_knn_ctx* ctx, int option, double value);
+
+/* Upload and centre the reference set; it stays resident in HBM.
+ * Replaces the reference-frame load/centre loop knn_rms.cpp:181-193
+ * (weights = topology masses, knn_rms.cpp:181-182). */
+int mdsctk_knn_set_reference(mdsctk_knn_ctx* ctx, const float* xyz_nm, int64_t n_ref,
+                             int32_t natoms, const float* weights);
+
+/* Same, from device memory (already in HBM). */
+int mdsctk_knn_set_reference_device(mdsctk_knn_ctx* ctx, const float* d_xyz_nm, int64_t n_ref,
+                                    int32_t natoms, const float* weights);
+
+/* k nearest reference frames of every query frame.
+ * Replaces the pair loops + permutation::sort + row writes
+ * (knn_rms.cpp:231-291, mdsctk.h:177-199).
+ *   xyz_nm   query frames [n_fit][natoms][3]; NULL queries the reference set
+ *            against itself (knn_rms without -f, knn_rms.cpp:103-104).
+ *   k        neighbours per row; clamped to n_ref-1 (knn_rms.cpp:224-225).
+ *   out_dist [n_fit][k] Angstrom, ascending; out_idx [n_fit][k] 0-based.
+ *            With MDSCTK_KNN_FULL_ROWS the outputs are [n_fit][n_ref] in
+ *            reference order and out_idx is the identity.
+ * Returns the number of entries written per row. */
+int mdsctk_knn_query(mdsctk_knn_ctx* ctx, const float* xyz_nm, int64_t n_fit, int32_t k,
+                     uint32_t flags, double* out_dist, int32_t* out_idx);
+
+/* Same with query frames and outputs in device memory (d_xyz_nm NULL = self);
+ * work is ordered on the context's stream and complete on return. */
+int mdsctk_knn_query_device(mdsctk_knn_ctx* ctx, const float* d_xyz_nm, int64_t n_fit,
+                            int32_t k, uint32_t flags, double* d_out_dist, int32_t* d_out_idx);
+
+/* Diagnostic (no counterpart in the reference): the approximate score
+ * s~ = G_r + G_f - 2 lambda~ (mass nm^2) that the contraction form computes
+ * for every pair, and optionally the exact FP64 score, both [n_fit][n_ref].
+ * Used to measure the contraction error that sizes the candidate window.
+ * Limited to n_fit*n_ref <= 6e7. */
+int mdsctk_knn_debug_scores(mdsctk_knn_ctx* ctx, const float* xyz_nm, int64_t n_fit,
+                            uint32_t flags, double* approx_s, double* exact_s);
+
+/* Statistics of the last completed query. */
+int mdsctk_knn_get_stats(const mdsctk_knn_ctx* ctx, mdsctk_knn_stats* out);
+
+/* Number of entries per row a query with this k and flags returns. */
+int mdsctk_knn_row_width(const mdsctk_knn_ctx* ctx, int32_t k, uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDSCTK_KNN_H */

@@ -1,0 +1,88 @@
+// Internal interface between the C-ABI layer (api.cpp) and the gfx950
+// kernels (knn_kernels.hip).  Not installed; include/mdsctk_knn.h is the
+// public surface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mdsctk {
+
+constexpr int kTile = 32;  // frames per MFMA tile edge (v_mfma_*_32x32xK)
+
+enum Prec : int { kFP32 = 0, kBF16 = 1, kFP16 = 2, kExact = 3 };
+
+// Atoms per operand K-group: split forms hold 16 atoms (one 32x32x16 MFMA
+// step) per group, FP32 holds 8 atoms (four 32x32x2 MFMA steps).
+inline int group_atoms(int prec) { return prec == kFP32 ? 8 : 16; }
+// 16-byte operand blocks of 64 lanes per (tile, K-group): 3 planes x parts.
+inline int blocks_per_group(int prec) { return prec == kFP32 ? 3 : 6; }
+
+// A frame set resident in HBM.  Row counts are padded to a multiple of
+// 2*kTile (the contraction walks tiles in pairs); padded rows are zero.
+struct DevFrames {
+  int64_t n = 0;        // real frames
+  int natoms = 0;
+  int n_tiles = 0;      // padded tile count (even)
+  int n_groups = 0;     // operand K-groups
+  int prec = kFP32;
+  float* xyz = nullptr;   // [n][natoms][3] as given (nm)
+  double* cen = nullptr;  // [n_tiles*32][3] FP64 mass-weighted centroid
+  double* G = nullptr;    // [n_tiles*32] sum w |x-c|^2, FP64
+  double* Gt = nullptr;   // [n_tiles*32] same sum over the rounded operand
+  uint4* op = nullptr;    // contraction operand, fragment-major (see prep kernel)
+  bool owns_xyz = true;
+};
+
+struct DevWeights {
+  int natoms = 0;
+  float* w = nullptr;   // masses
+  float* sw = nullptr;  // sqrt(mass)
+  double W = 0;         // sum of masses
+};
+
+// Per-query candidate lists written by the contraction kernel.
+struct Lists {
+  int nl = 0;          // lists per query
+  int cap = 0;         // entries per list
+  uint2* ent = nullptr;       // [n_fit][nl][cap] (float key bits, ref index)
+  int* cnt = nullptr;         // [n_fit][nl]
+  float* ovf = nullptr;       // [n_fit][nl] smallest key dropped on overflow
+  unsigned* thr = nullptr;    // [n_fit] shared threshold (float bits)
+};
+
+// Launchers (all asynchronous on `st`).
+hipError_t launch_prep(const DevFrames& f, const DevWeights& w, hipStream_t st);
+hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st);
+// eps2[q] = 2 kappa (Gt_fit[q] + max Gt_ref); gmax is one device double of scratch.
+hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, float* eps2,
+                      double* gmax, hipStream_t st);
+// Approximate contraction; when dbg_scores != nullptr it instead writes the
+// approximate score of every pair to dbg_scores[fit*n_ref+ref] (small sizes).
+hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lists& L,
+                           const float* eps2, int k1, int ref_phases, bool nofit,
+                           double* dbg_scores, int grid_wgs, hipStream_t st);
+hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
+                        const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
+                        bool nofit, double* out_dist, int* out_idx, int* fallback,
+                        int* n_cand, hipStream_t st);
+// Exact FP64 scores s = G_a + G_b - 2 lambda for fit rows [row0,row0+nrows)
+// against all refs.  Writes s into srow[(row-row0)*n_ref + ref] unless
+// out_full is given, in which case it writes Angstrom distances into
+// out_full[row*n_ref + ref] (the --sort false layout).
+hipError_t launch_exact_rows(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
+                             const int* rows, int nrows, bool nofit, double* srow,
+                             double* out_full, int64_t out_ld, hipStream_t st);
+// Top-k1 by (s, index) over exact score rows; writes ranks
+// [rank_off, rank_off+k_out) of each listed fit row.
+hipError_t launch_select_rows(const double* srow, int64_t n_ref, const int* rows, int nrows,
+                              double W, int k1, int k_out, int rank_off, double* out_dist,
+                              int* out_idx, hipStream_t st);
+hipError_t launch_identity_rows(int64_t n_fit, int64_t n_ref, int* out_idx, hipStream_t st);
+hipError_t launch_max_gt(const DevFrames& f, double* out, hipStream_t st);
+
+// Choose the per-list capacity for k1 (0 when the fused path cannot serve it).
+int list_capacity_for(int k1);
+
+}  // namespace mdsctk

@@ -1,0 +1,1206 @@
+// gfx950 kernels of the knn_rms engine.
+//
+// Pipeline for one query batch (reference: knn_rms.cpp:231-291):
+//   prep_kernel      one pass per 32-frame tile: FP64 mass-weighted centroid,
+//                    G = sum w|x-c|^2 (FP64), and the contraction operand
+//                    sqrt(w)(x-c) rounded to FP32 or split into hi/lo 16-bit
+//                    halves, written fragment-major (below).  HBM-bound.
+//   contract_kernel  all-pairs 3x3 cross-covariance blocks as one dense MFMA
+//                    contraction (9 accumulator tiles per wave: planes c,d of
+//                    32 ref x 32 fit frames), FP64 QCP Newton epilogue giving
+//                    an approximate score s~ = G_r + G_f - 2 lambda_max, and
+//                    per-lane candidate lists (threshold + append, wave
+//                    bitonic compaction).  No distance matrix is written.
+//   merge_kernel     per query: radix-select the (k+1)-th smallest s~ over
+//                    its lists, take every candidate within the certified
+//                    window, re-score them exactly in FP64 from the raw
+//                    coordinates, sort by (s, index), write ranks 1..k.
+//   exact_rows / select_rows
+//                    FP64 scores of whole rows: overflow fallback, the
+//                    MDSCTK_PREC_EXACT form, and --sort false full rows.
+//
+// Operand layout ("fragment-major"): for tile T (32 frames), K-group g and
+// block b (plane c, part), 64 lanes x 16 bytes are contiguous, laid out
+// exactly as the MFMA A/B operand registers want them:
+//   split (32x32x16 bf16/f16): lane l holds frame l%32, atoms 16g+8(l/32)+j,
+//                              j=0..7; b = 2c + part (part 0 = hi, 1 = lo)
+//   fp32  (32x32x2 f32):       lane l holds frame l%32, atoms 8g+2j+(l/32),
+//                              j=0..3 (one per MFMA step); b = c
+// so every global load and every LDS read of a fragment is one contiguous
+// 1 KB access per wave.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+#include "knn_device.h"
+
+namespace mdsctk {
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+constexpr unsigned kInfBits = 0x7f800000u;
+constexpr int kCandMax = 2048;
+
+// ---------------------------------------------------------------------------
+// small helpers
+
+__device__ __forceinline__ unsigned short bf16_rne(float v) {
+  unsigned u = __float_as_uint(v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ double bf16_val(unsigned short h) {
+  return (double)__uint_as_float(((unsigned)h) << 16);
+}
+__device__ __forceinline__ unsigned short f16_rne(float v) {
+  _Float16 h = (_Float16)v;
+  return __builtin_bit_cast(unsigned short, h);
+}
+__device__ __forceinline__ double f16_val(unsigned short h) {
+  return (double)(float)__builtin_bit_cast(_Float16, h);
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ double det3(const double* m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+         m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+// Horn key matrix entries of S (row-major 3x3) and the coefficients of its
+// characteristic polynomial l^4 + c2 l^2 + c1 l + c0.
+__device__ __forceinline__ void qcp_coeffs(const double* S, double& c2, double& c1, double& c0) {
+  const double xx = S[0], xy = S[1], xz = S[2];
+  const double yx = S[3], yy = S[4], yz = S[5];
+  const double zx = S[6], zy = S[7], zz = S[8];
+  const double k00 = xx + yy + zz, k01 = yz - zy, k02 = zx - xz, k03 = xy - yx;
+  const double k11 = xx - yy - zz, k12 = xy + yx, k13 = zx + xz;
+  const double k22 = -xx + yy - zz, k23 = yz + zy;
+  const double k33 = -xx - yy + zz;
+  // det(K) by 2x2 minors of rows (0,1) and (2,3); K symmetric.
+  const double s0 = k00 * k11 - k01 * k01;
+  const double s1 = k00 * k12 - k01 * k02;
+  const double s2 = k00 * k13 - k01 * k03;
+  const double s3 = k01 * k12 - k11 * k02;
+  const double s4 = k01 * k13 - k11 * k03;
+  const double s5 = k02 * k13 - k12 * k03;
+  const double t5 = k22 * k33 - k23 * k23;
+  const double t4 = k12 * k33 - k13 * k23;
+  const double t3 = k12 * k23 - k13 * k22;
+  const double t2 = k02 * k33 - k03 * k23;
+  const double t1 = k02 * k23 - k03 * k22;
+  const double t0 = k02 * k13 - k03 * k12;
+  c0 = s0 * t5 - s1 * t4 + s2 * t3 + s3 * t2 - s4 * t1 + s5 * t0;
+  double fro = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) fro += S[i] * S[i];
+  c2 = -2.0 * fro;
+  c1 = -8.0 * det3(S);
+}
+
+// Largest eigenvalue of a symmetric 4x4 matrix by cyclic Jacobi (used only
+// when Newton does not converge, e.g. near-linear frames).
+__device__ double jacobi4_max(const double* S) {
+  double a[4][4];
+  {
+    const double xx = S[0], xy = S[1], xz = S[2], yx = S[3], yy = S[4], yz = S[5];
+    const double zx = S[6], zy = S[7], zz = S[8];
+    a[0][0] = xx + yy + zz; a[0][1] = yz - zy; a[0][2] = zx - xz; a[0][3] = xy - yx;
+    a[1][1] = xx - yy - zz; a[1][2] = xy + yx; a[1][3] = zx + xz;
+    a[2][2] = -xx + yy - zz; a[2][3] = yz + zy; a[3][3] = -xx - yy + zz;
+    a[1][0] = a[0][1]; a[2][0] = a[0][2]; a[3][0] = a[0][3];
+    a[2][1] = a[1][2]; a[3][1] = a[1][3]; a[3][2] = a[2][3];
+  }
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < 3; ++p)
+      for (int q = p + 1; q < 4; ++q) off += fabs(a[p][q]);
+    if (off < 1e-300) break;
+    for (int p = 0; p < 3; ++p)
+      for (int q = p + 1; q < 4; ++q) {
+        const double apq = a[p][q];
+        if (apq == 0.0) continue;
+        const double th = (a[q][q] - a[p][p]) / (2.0 * apq);
+        double t = 1.0 / (fabs(th) + sqrt(th * th + 1.0));
+        if (th < 0.0) t = -t;
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 4; ++k) {
+          const double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - s * akq;
+          a[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 4; ++k) {
+          const double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - s * aqk;
+          a[q][k] = s * apk + c * aqk;
+        }
+      }
+  }
+  return fmax(fmax(a[0][0], a[1][1]), fmax(a[2][2], a[3][3]));
+}
+
+// Exact lambda_max of K(S) for the FP64 re-score.
+__device__ double qcp_lambda_exact(const double* S_in, double ga, double gb) {
+  const double g = 0.5 * (ga + gb);
+  if (!(g > 0.0)) return 0.0;
+  double S[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = S_in[i] / g;
+  double c2, c1, c0;
+  qcp_coeffs(S, c2, c1, c0);
+  double l = 1.0;
+  bool conv = false;
+  for (int it = 0; it < 100; ++it) {
+    const double l2 = l * l;
+    const double p = (l2 + c2) * l2 + c1 * l + c0;
+    const double dp = (4.0 * l2 + 2.0 * c2) * l + c1;
+    if (dp == 0.0) break;
+    const double step = p / dp;
+    l -= step;
+    if (fabs(step) <= 1e-15 * fabs(l)) { conv = true; break; }
+  }
+  if (!conv) l = jacobi4_max(S);
+  return l * g;
+}
+
+// Approximate score for candidate selection.  Newton from the upper bound
+// l = 1 (i.e. (G_a+G_b)/2) decreases monotonically to lambda_max, so every
+// iterate gives a lower bound of s~; once that bound exceeds `reject` the
+// pair cannot be a candidate and iteration stops.
+__device__ __forceinline__ double qcp_score_approx(const float* m, double ga, double gb,
+                                                   double reject, bool nofit) {
+  if (nofit) return ga + gb - 2.0 * ((double)m[0] + (double)m[4] + (double)m[8]);
+  const double g = 0.5 * (ga + gb);
+  if (!(g > 0.0)) return 0.0;
+  const double ig = 1.0 / g;
+  double S[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = (double)m[i] * ig;
+  double c2, c1, c0;
+  qcp_coeffs(S, c2, c1, c0);
+  const double rej = 0.5 * reject * ig;
+  double l = 1.0;
+  for (int it = 0; it < 24; ++it) {
+    const double l2 = l * l;
+    const double p = (l2 + c2) * l2 + c1 * l + c0;
+    const double dp = (4.0 * l2 + 2.0 * c2) * l + c1;
+    const double step = p * __builtin_amdgcn_rcp(dp);
+    l -= step;
+    if (1.0 - l > rej) break;
+    if (fabs(step) < 1e-11) break;
+  }
+  const double s = 2.0 * g * (1.0 - l);
+  // A non-finite iterate (degenerate input) must not hide a pair: report 0
+  // so that it becomes a candidate and is re-scored exactly.
+  return (s == s && s < INFINITY) ? s : 0.0;
+}
+
+// Wave-level bitonic sort, ascending by (key, val), of 64*PER elements held
+// PER per lane with element index e = lane*PER + j.
+__device__ __forceinline__ bool kv_less(unsigned ka, int va, unsigned kb, int vb) {
+  return ka < kb || (ka == kb && va < vb);
+}
+
+template <int PER>
+__device__ __forceinline__ void wave_sort(unsigned (&key)[PER], int (&val)[PER]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int size = 2; size <= 64 * PER; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= PER) {
+        const int ls = stride / PER;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+          const int e = lane * PER + j;
+          const unsigned ok = __shfl_xor(key[j], ls);
+          const int ov = __shfl_xor(val[j], ls);
+          const bool up = (e & size) == 0;
+          const bool lower = (e & stride) == 0;
+          const bool take_other = (lower == up) ? kv_less(ok, ov, key[j], val[j])
+                                                : kv_less(key[j], val[j], ok, ov);
+          if (take_other) { key[j] = ok; val[j] = ov; }
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+          const int jp = j ^ stride;
+          if (jp > j) {
+            const int e = lane * PER + j;
+            const bool up = (e & size) == 0;
+            const bool sw = up ? kv_less(key[jp], val[jp], key[j], val[j])
+                               : kv_less(key[j], val[j], key[jp], val[jp]);
+            if (sw) {
+              const unsigned tk = key[j]; key[j] = key[jp]; key[jp] = tk;
+              const int tv = val[j]; val[j] = val[jp]; val[jp] = tv;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int PER>
+__device__ __forceinline__ unsigned pick(const unsigned (&a)[PER], int j) {
+  unsigned r = a[0];
+#pragma unroll
+  for (int i = 1; i < PER; ++i)
+    if (i == j) r = a[i];
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// prep: one 192-thread workgroup per 32-frame tile.  Thread t -> plane
+// c = t/64, lane l = t%64 of that plane's operand block.
+
+template <int PREC>
+__global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz, int64_t n,
+                                                   int natoms, int n_groups,
+                                                   const float* __restrict__ w,
+                                                   const float* __restrict__ sw, double Wtot,
+                                                   double* __restrict__ cen,
+                                                   double* __restrict__ G,
+                                                   double* __restrict__ Gt,
+                                                   uint4* __restrict__ op) {
+  constexpr int KG = (PREC == kFP32) ? 8 : 16;
+  constexpr int NE = (PREC == kFP32) ? 4 : 8;
+  constexpr int CH = KG * 3;
+  __shared__ float xs[kTile * CH];
+  __shared__ double red[192];
+  __shared__ double scen[kTile * 3];
+
+  const int tile = blockIdx.x;
+  const int t = threadIdx.x;
+  const int c = t >> 6;
+  const int lane = t & 63;
+  const int row = lane & 31;
+  const int h = lane >> 5;
+
+  auto stage = [&](int g) {
+    for (int i = t; i < kTile * CH; i += 192) {
+      const int fr = i / CH, e = i - fr * CH;
+      const int64_t ff = (int64_t)tile * kTile + fr;
+      const int a = g * KG + e / 3;
+      float v = 0.f;
+      if (ff < n && a < natoms) v = xyz[(ff * natoms + (int64_t)g * KG) * 3 + e];
+      xs[i] = v;
+    }
+  };
+  auto local_atom = [&](int j) { return (PREC == kFP32) ? (2 * j + h) : (8 * h + j); };
+
+  double acc = 0.0;
+  for (int g = 0; g < n_groups; ++g) {
+    __syncthreads();
+    stage(g);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int la = local_atom(j);
+      const int a = g * KG + la;
+      if (a < natoms) acc += (double)w[a] * (double)xs[row * CH + la * 3 + c];
+    }
+  }
+  red[t] = acc;
+  __syncthreads();
+  if (t < 96) {
+    const int cc = t >> 5, rr = t & 31;
+    scen[rr * 3 + cc] = (red[cc * 64 + rr] + red[cc * 64 + 32 + rr]) / Wtot;
+  }
+  __syncthreads();
+  const double cm = scen[row * 3 + c];
+
+  double gex = 0.0, gop = 0.0;
+  for (int g = 0; g < n_groups; ++g) {
+    __syncthreads();
+    stage(g);
+    __syncthreads();
+    if constexpr (PREC == kFP32) {
+      float v4[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int la = local_atom(j);
+        const int a = g * KG + la;
+        float v = 0.f;
+        if (a < natoms) {
+          const double d = (double)xs[row * CH + la * 3 + c] - cm;
+          gex += (double)w[a] * d * d;
+          v = (float)((double)sw[a] * d);
+        }
+        v4[j] = v;
+        gop += (double)v * (double)v;
+      }
+      uint4 o;
+      o.x = __float_as_uint(v4[0]);
+      o.y = __float_as_uint(v4[1]);
+      o.z = __float_as_uint(v4[2]);
+      o.w = __float_as_uint(v4[3]);
+      op[(((int64_t)tile * n_groups + g) * 3 + c) * 64 + lane] = o;
+    } else {
+      unsigned short hi[8], lo[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int la = local_atom(j);
+        const int a = g * KG + la;
+        double v = 0.0;
+        if (a < natoms) {
+          const double d = (double)xs[row * CH + la * 3 + c] - cm;
+          gex += (double)w[a] * d * d;
+          v = (double)sw[a] * d;
+        }
+        double vh, vl;
+        if constexpr (PREC == kBF16) {
+          hi[j] = bf16_rne((float)v);
+          vh = bf16_val(hi[j]);
+          lo[j] = bf16_rne((float)(v - vh));
+          vl = bf16_val(lo[j]);
+        } else {
+          hi[j] = f16_rne((float)v);
+          vh = f16_val(hi[j]);
+          lo[j] = f16_rne((float)(v - vh));
+          vl = f16_val(lo[j]);
+        }
+        gop += vh * vh + 2.0 * vh * vl;
+      }
+      uint4 oh, ol;
+      oh.x = hi[0] | ((unsigned)hi[1] << 16); oh.y = hi[2] | ((unsigned)hi[3] << 16);
+      oh.z = hi[4] | ((unsigned)hi[5] << 16); oh.w = hi[6] | ((unsigned)hi[7] << 16);
+      ol.x = lo[0] | ((unsigned)lo[1] << 16); ol.y = lo[2] | ((unsigned)lo[3] << 16);
+      ol.z = lo[4] | ((unsigned)lo[5] << 16); ol.w = lo[6] | ((unsigned)lo[7] << 16);
+      const int64_t blk = (((int64_t)tile * n_groups + g) * 6 + 2 * c);
+      op[blk * 64 + lane] = oh;
+      op[(blk + 1) * 64 + lane] = ol;
+    }
+  }
+  __syncthreads();
+  red[t] = gex;
+  __syncthreads();
+  if (t < 32) {
+    double s = 0.0;
+    for (int cc = 0; cc < 3; ++cc) s += red[cc * 64 + t] + red[cc * 64 + 32 + t];
+    G[(int64_t)tile * kTile + t] = s;
+  }
+  __syncthreads();
+  red[t] = gop;
+  __syncthreads();
+  if (t < 32) {
+    double s = 0.0;
+    for (int cc = 0; cc < 3; ++cc) s += red[cc * 64 + t] + red[cc * 64 + 32 + t];
+    const int64_t f = (int64_t)tile * kTile + t;
+    Gt[f] = s;
+    cen[f * 3 + 0] = scen[t * 3 + 0];
+    cen[f * 3 + 1] = scen[t * 3 + 1];
+    cen[f * 3 + 2] = scen[t * 3 + 2];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// contraction + QCP + candidate lists
+
+struct ContractParams {
+  const uint4* rop;
+  const uint4* fop;
+  const double* rGt;
+  const double* fGt;
+  int64_t n_ref, n_fit;
+  int ref_pairs, fit_blocks, n_groups, phases, k1;
+  const float* eps2;
+  uint2* ent;
+  int* cnt;
+  float* ovf;
+  unsigned* thr;
+  int nl, cap;
+  int nofit;
+  double* dbg;
+};
+
+// One K-group of MFMA work for a wave: acc[c*3+d] += A_c * B_d^T over the
+// group's atoms, reading the wave's fragments from LDS.  For the split forms
+// the three products hi*hi + hi*lo + lo*hi accumulate into the same FP32
+// accumulators (the lo*lo term is dropped).
+template <int PREC>
+__device__ __forceinline__ void mma_group(f32x16 (&acc)[9], const uint4* __restrict__ As,
+                                          const uint4* __restrict__ Bs, int lane) {
+  if constexpr (PREC == kFP32) {
+    uint4 b[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) b[d] = Bs[d * 64 + lane];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const uint4 a = As[c * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float av = __uint_as_float(j == 0 ? a.x : j == 1 ? a.y : j == 2 ? a.z : a.w);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float bv =
+              __uint_as_float(j == 0 ? b[d].x : j == 1 ? b[d].y : j == 2 ? b[d].z : b[d].w);
+          acc[c * 3 + d] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[c * 3 + d], 0, 0, 0);
+        }
+      }
+    }
+  } else {
+    uint4 bh[3], bl[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      bh[d] = Bs[(2 * d) * 64 + lane];
+      bl[d] = Bs[(2 * d + 1) * 64 + lane];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const uint4 ah = As[(2 * c) * 64 + lane];
+      const uint4 al = As[(2 * c + 1) * 64 + lane];
+#pragma unroll
+      for (int prod = 0; prod < 3; ++prod) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const uint4 av = (prod == 2) ? al : ah;
+          const uint4 bv = (prod == 1) ? bl[d] : bh[d];
+          if constexpr (PREC == kBF16) {
+            acc[c * 3 + d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                __builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv), acc[c * 3 + d], 0,
+                0, 0);
+          } else {
+            acc[c * 3 + d] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                __builtin_bit_cast(f16x8, av), __builtin_bit_cast(f16x8, bv), acc[c * 3 + d], 0,
+                0, 0);
+          }
+        }
+      }
+    }
+  }
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// Wave `slot` copies its tile's K-group g (BPG blocks of 64 x 16 B) from HBM
+// straight into LDS (global_load_lds_dwordx4: no VGPR staging).
+template <int BPG>
+__device__ __forceinline__ void dma_group(const uint4* __restrict__ src_tile, int g,
+                                          uint4* dst_slot, int lane) {
+  const uint4* src = src_tile + (int64_t)g * (BPG * 64) + lane;
+#pragma unroll
+  for (int j = 0; j < BPG; ++j)
+    __builtin_amdgcn_global_load_lds((const void*)(src + j * 64), (lds_void_t*)(dst_slot + j * 64),
+                                     16, 0, 0);
+}
+
+// Compact the lists of every lane whose list is nearly full.  Each list is
+// sorted by the whole wave, its (k1)-th smallest key tau found, and only
+// entries <= tau + 2 eps kept (at most cap-64 of them; the smallest dropped
+// key is remembered in `ovf`).
+template <int PER>
+__device__ __attribute__((noinline)) void compact_lists(uint2* __restrict__ ent, int64_t my_base, int& cnt,
+                                              float& thr, float& ovf, float eps2, int k1,
+                                              int cap, bool active) {
+  const int lane = threadIdx.x & 63;
+  uint64_t need = __ballot(active && cnt > cap - 16);
+  while (need) {
+    const int L = __builtin_ctzll(need);
+    need &= need - 1;
+    const int64_t base = __shfl(my_base, L);
+    const int n = __shfl(cnt, L);
+    const float e2 = __shfl(eps2, L);
+    unsigned key[PER];
+    int val[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int e = lane * PER + j;
+      if (e < n) {
+        const uint2 x = ent[base + e];
+        key[j] = x.x;
+        val[j] = (int)x.y;
+      } else {
+        key[j] = kInfBits;
+        val[j] = INT_MAX;
+      }
+    }
+    wave_sort<PER>(key, val);
+    const int tr = k1 - 1;
+    const unsigned tk = __shfl(pick<PER>(key, tr % PER), tr / PER);
+    const float tau = (n >= k1) ? __uint_as_float(tk) : INFINITY;
+    const float cut = tau * (1.0f + 2.4e-7f) + e2;
+    int mine = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int e = lane * PER + j;
+      mine += (e < n && __uint_as_float(key[j]) <= cut) ? 1 : 0;
+    }
+    int keep = wave_sum_i(mine);
+    const int kcap = cap - 64;
+    float dropped = INFINITY;
+    if (keep > kcap) {
+      dropped = __uint_as_float(__shfl(pick<PER>(key, kcap % PER), kcap / PER));
+      keep = kcap;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int e = lane * PER + j;
+      if (e < keep) ent[base + e] = make_uint2(key[j], (unsigned)val[j]);
+    }
+    if (lane == L) {
+      cnt = keep;
+      thr = fminf(thr, cut);
+      if (dropped < INFINITY) {
+        ovf = fminf(ovf, dropped);
+        thr = fminf(thr, dropped);
+      }
+    }
+  }
+}
+
+template <int PREC, int PER>
+__global__ __launch_bounds__(256, 2) void contract_kernel(ContractParams P) {
+  constexpr int BPG = (PREC == kFP32) ? 3 : 6;
+  constexpr int TILE_VEC = BPG * 64;
+  constexpr int STAGE_VEC = 4 * TILE_VEC;
+  // Two stage buffers as distinct LDS objects, so that the compiler's
+  // LDS-DMA dependency tracking lets reads of one overlap copies into the
+  // other.  Stage layout: [ref tile 0][ref tile 1][fit tile 0][fit tile 1].
+  __shared__ uint4 sbuf0[STAGE_VEC];
+  __shared__ uint4 sbuf1[STAGE_VEC];
+  __shared__ double sGr[2 * kTile];
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int ri = wave & 1;
+  const int fi = wave >> 1;
+
+  // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
+  // that consecutive work items (same fit block, neighbouring reference
+  // phases) land on the same XCD and share its L2.
+  const int nwg = gridDim.x;
+  const int logical = (int)(blockIdx.x & 7) * (nwg >> 3) + (int)(blockIdx.x >> 3);
+  const int n_items = P.fit_blocks * P.phases;
+
+  for (int item = logical; item < n_items; item += nwg) {
+    const int fb = item / P.phases;
+    const int ph = item - fb * P.phases;
+    const int64_t fit_idx = (int64_t)(2 * fb + fi) * kTile + (lane & 31);
+    const bool fvalid = fit_idx < P.n_fit;
+    const int list_id = (ph * 2 + ri) * 2 + (lane >> 5);
+    const int64_t hdr = fit_idx * P.nl + list_id;
+    const int64_t my_base = hdr * P.cap;
+    const double gf = P.fGt[fit_idx];
+    const float e2 = (fvalid && P.eps2) ? P.eps2[fit_idx] : 0.f;
+    int cnt = 0;
+    float thr = INFINITY, ovf = INFINITY;
+
+    for (int p = ph; p < P.ref_pairs; p += P.phases) {
+      if (fvalid && P.thr) thr = fminf(thr, __uint_as_float(__builtin_nontemporal_load(&P.thr[fit_idx])));
+      f32x16 acc[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] = (f32x16){0};
+
+      // Wave w copies stage slot w: ref tiles 2p, 2p+1, fit tiles 2fb, 2fb+1.
+      const uint4* src_tile =
+          (wave < 2) ? P.rop + (int64_t)(2 * p + wave) * P.n_groups * TILE_VEC
+                     : P.fop + (int64_t)(2 * fb + wave - 2) * P.n_groups * TILE_VEC;
+      __syncthreads();
+      if (tid < 2 * kTile) sGr[tid] = P.rGt[(int64_t)p * 2 * kTile + tid];
+      dma_group<BPG>(src_tile, 0, sbuf0 + wave * TILE_VEC, lane);
+      __syncthreads();
+      int g = 0;
+      for (; g + 1 < P.n_groups; g += 2) {
+        dma_group<BPG>(src_tile, g + 1, sbuf1 + wave * TILE_VEC, lane);
+        mma_group<PREC>(acc, sbuf0 + ri * TILE_VEC, sbuf0 + (2 + fi) * TILE_VEC, lane);
+        __syncthreads();
+        if (g + 2 < P.n_groups) dma_group<BPG>(src_tile, g + 2, sbuf0 + wave * TILE_VEC, lane);
+        mma_group<PREC>(acc, sbuf1 + ri * TILE_VEC, sbuf1 + (2 + fi) * TILE_VEC, lane);
+        __syncthreads();
+      }
+      if (g < P.n_groups) {
+        mma_group<PREC>(acc, sbuf0 + ri * TILE_VEC, sbuf0 + (2 + fi) * TILE_VEC, lane);
+        __syncthreads();
+      }
+
+      // Epilogue: lane holds fit frame (lane%32) against 16 ref frames.
+      const int64_t ref_base = (int64_t)(2 * p + ri) * kTile;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int rl = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t ref_idx = ref_base + rl;
+        if (!fvalid || ref_idx >= P.n_ref) continue;
+        float m[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = acc[q][r];
+        const double gr = sGr[ri * kTile + rl];
+        if (P.dbg) {
+          P.dbg[fit_idx * P.n_ref + ref_idx] = qcp_score_approx(m, gr, gf, INFINITY, P.nofit);
+          continue;
+        }
+        const double s = qcp_score_approx(m, gr, gf, (double)thr, P.nofit);
+        if (s <= (double)thr) {
+          const float key = (float)fmax(s, 0.0);
+          P.ent[my_base + cnt] = make_uint2(__float_as_uint(key), (unsigned)ref_idx);
+          ++cnt;
+        }
+      }
+      if (P.dbg) continue;
+      const float thr_before = thr;
+      compact_lists<PER>(P.ent, my_base, cnt, thr, ovf, e2, P.k1, P.cap, fvalid);
+      thr = fminf(thr, __shfl_xor(thr, 32));
+      if (fvalid && P.thr && thr < thr_before) atomicMin(&P.thr[fit_idx], __float_as_uint(thr));
+    }
+    if (fvalid && !P.dbg) {
+      P.cnt[hdr] = cnt;
+      P.ovf[hdr] = ovf;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact FP64 score of one pair, cooperatively by a wave (lanes over atoms).
+
+__device__ __forceinline__ double wave_exact_score(const float* __restrict__ a,
+                                                   const double* __restrict__ ca, double ga,
+                                                   const float* __restrict__ b,
+                                                   const double* __restrict__ cb, double gb,
+                                                   const float* __restrict__ w, int natoms,
+                                                   bool nofit) {
+  const int lane = threadIdx.x & 63;
+  double S[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = 0.0;
+  const double ca0 = ca[0], ca1 = ca[1], ca2 = ca[2];
+  const double cb0 = cb[0], cb1 = cb[1], cb2 = cb[2];
+  for (int i = lane; i < natoms; i += 64) {
+    const double wi = (double)w[i];
+    const double ax = (double)a[3 * i] - ca0, ay = (double)a[3 * i + 1] - ca1,
+                 az = (double)a[3 * i + 2] - ca2;
+    const double bx = (double)b[3 * i] - cb0, by = (double)b[3 * i + 1] - cb1,
+                 bz = (double)b[3 * i + 2] - cb2;
+    const double wx = wi * ax, wy = wi * ay, wz = wi * az;
+    S[0] += wx * bx; S[1] += wx * by; S[2] += wx * bz;
+    S[3] += wy * bx; S[4] += wy * by; S[5] += wy * bz;
+    S[6] += wz * bx; S[7] += wz * by; S[8] += wz * bz;
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = wave_sum_d(S[i]);
+  const double lam = nofit ? (S[0] + S[4] + S[8]) : qcp_lambda_exact(S, ga, gb);
+  return fmax(ga + gb - 2.0 * lam, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// merge: radix select over the lists, certified window, FP64 re-score.
+
+struct MergeParams {
+  const float* rxyz;
+  const double* rcen;
+  const double* rG;
+  const float* fxyz;
+  const double* fcen;
+  const double* fG;
+  const float* w;
+  int natoms;
+  double W;
+  const uint2* ent;
+  const int* cnt;
+  const float* ovf;
+  int nl, cap;
+  const float* eps2;
+  int k1, k_out, rank_off;
+  int nofit;
+  double* out_dist;
+  int* out_idx;
+  int* fallback;
+  int* n_cand;
+};
+
+__device__ void block_bitonic_sort(double* s, int* idx, int n_pow2) {
+  const int tid = threadIdx.x;
+  for (int size = 2; size <= n_pow2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int i = tid; i < n_pow2 / 2; i += blockDim.x) {
+        const int lo = 2 * stride * (i / stride) + (i % stride);
+        const int hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const bool gt = (s[lo] > s[hi]) || (s[lo] == s[hi] && idx[lo] > idx[hi]);
+        if (gt == up) {
+          const double ts = s[lo]; s[lo] = s[hi]; s[hi] = ts;
+          const int ti = idx[lo]; idx[lo] = idx[hi]; idx[hi] = ti;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
+  const int64_t q = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  __shared__ unsigned hist[256];
+  __shared__ unsigned s_prefix;
+  __shared__ int s_kk, s_nc, s_flag, s_total;
+  __shared__ int cand_idx[kCandMax];
+  __shared__ double cand_s[kCandMax];
+
+  const int* cnt = P.cnt + q * P.nl;
+  const uint2* ent = P.ent + q * P.nl * (int64_t)P.cap;
+  const float* ovf = P.ovf + q * P.nl;
+
+  if (tid == 0) { s_total = 0; s_nc = 0; s_flag = 0; }
+  __syncthreads();
+  {
+    int part = 0;
+    for (int l = tid; l < P.nl; l += 256) part += cnt[l];
+    atomicAdd(&s_total, part);
+  }
+  __syncthreads();
+  if (s_total < P.k1) {
+    if (tid == 0) P.fallback[q] = 1;
+    return;
+  }
+
+  unsigned prefix = 0;
+  int kk = P.k1 - 1;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    const unsigned mhi = (shift == 24) ? 0u : (0xFFFFFFFFu << (shift + 8));
+    for (int l = wave; l < P.nl; l += 4) {
+      const int c = cnt[l];
+      for (int i = lane; i < c; i += 64) {
+        const unsigned k = ent[(int64_t)l * P.cap + i].x;
+        if ((k & mhi) == (prefix & mhi)) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned cum = 0;
+      int b = 0;
+      for (; b < 255; ++b) {
+        if (cum + hist[b] > (unsigned)kk) break;
+        cum += hist[b];
+      }
+      s_prefix = prefix | ((unsigned)b << shift);
+      s_kk = kk - (int)cum;
+    }
+    __syncthreads();
+    prefix = s_prefix;
+    kk = s_kk;
+  }
+  const float tau = __uint_as_float(prefix);
+  const float cut = tau * (1.0f + 2.4e-7f) + P.eps2[q];
+  for (int l = tid; l < P.nl; l += 256)
+    if (ovf[l] <= cut) s_flag = 1;
+  for (int l = wave; l < P.nl; l += 4) {
+    const int c = cnt[l];
+    for (int i = lane; i < c; i += 64) {
+      const uint2 e = ent[(int64_t)l * P.cap + i];
+      if (__uint_as_float(e.x) <= cut) {
+        const int pos = atomicAdd(&s_nc, 1);
+        if (pos < kCandMax) cand_idx[pos] = (int)e.y;
+      }
+    }
+  }
+  __syncthreads();
+  const int nc = s_nc;
+  if (s_flag || nc > kCandMax) {
+    if (tid == 0) P.fallback[q] = 1;
+    return;
+  }
+  if (tid == 0) {
+    P.fallback[q] = 0;
+    atomicAdd(P.n_cand, nc);
+  }
+
+  const float* fx = P.fxyz + q * (int64_t)P.natoms * 3;
+  for (int c = wave; c < nc; c += 4) {
+    const int64_t r = cand_idx[c];
+    const double s = wave_exact_score(P.rxyz + r * (int64_t)P.natoms * 3, P.rcen + r * 3, P.rG[r],
+                                      fx, P.fcen + q * 3, P.fG[q], P.w, P.natoms, P.nofit);
+    if (lane == 0) cand_s[c] = s;
+  }
+  int np2 = 64;
+  while (np2 < nc) np2 <<= 1;
+  for (int i = nc + tid; i < np2; i += 256) {
+    cand_s[i] = INFINITY;
+    cand_idx[i] = INT_MAX;
+  }
+  block_bitonic_sort(cand_s, cand_idx, np2);
+  for (int j = tid; j < P.k_out; j += 256) {
+    const int src = j + P.rank_off;
+    P.out_dist[q * P.k_out + j] = 10.0 * sqrt(fmax(cand_s[src], 0.0) / P.W);
+    P.out_idx[q * P.k_out + j] = cand_idx[src];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact rows
+
+struct ExactParams {
+  const float* rxyz;
+  const double* rcen;
+  const double* rG;
+  const float* fxyz;
+  const double* fcen;
+  const double* fG;
+  const float* w;
+  int natoms;
+  double W;
+  int64_t n_ref;
+  const int* rows;
+  int64_t row0;
+  int nofit;
+  double* srow;
+  double* out_full;
+  int64_t out_ld;
+};
+
+constexpr int kExactRefsPerWG = 64;
+
+__global__ __launch_bounds__(256) void exact_rows_kernel(ExactParams P) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t ri = blockIdx.y;
+  const int64_t q = P.rows ? P.rows[ri] : P.row0 + ri;
+  const float* fx = P.fxyz + q * (int64_t)P.natoms * 3;
+  const int64_t r0 = (int64_t)blockIdx.x * kExactRefsPerWG;
+  for (int j = wave; j < kExactRefsPerWG; j += 4) {
+    const int64_t r = r0 + j;
+    if (r >= P.n_ref) break;
+    const double s = wave_exact_score(P.rxyz + r * (int64_t)P.natoms * 3, P.rcen + r * 3, P.rG[r],
+                                      fx, P.fcen + q * 3, P.fG[q], P.w, P.natoms, P.nofit);
+    if (lane == 0) {
+      if (P.out_full)
+        P.out_full[q * P.out_ld + r] = 10.0 * sqrt(s / P.W);
+      else
+        P.srow[ri * P.n_ref + r] = s;
+    }
+  }
+}
+
+// Top-k1 by (s, index) over one exact score row per workgroup.
+__global__ __launch_bounds__(256) void select_rows_kernel(const double* __restrict__ srow,
+                                                          int64_t n_ref,
+                                                          const int* __restrict__ rows,
+                                                          double W, int k1, int k_out,
+                                                          int rank_off, double* out_dist,
+                                                          int* out_idx) {
+  const int64_t ri = blockIdx.x;
+  const int64_t q = rows ? rows[ri] : ri;
+  const double* s = srow + ri * n_ref;
+  const int tid = threadIdx.x;
+  __shared__ unsigned hist[256];
+  __shared__ unsigned long long s_prefix;
+  __shared__ int s_kk, s_nc;
+  __shared__ unsigned s_iprefix;
+  __shared__ int cand_idx[kCandMax];
+  __shared__ double cand_s[kCandMax];
+
+  // rank k1-1 among the 64-bit keys (s >= 0, so the bit pattern is monotone)
+  unsigned long long prefix = 0;
+  int kk = k1 - 1;
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    const unsigned long long mhi = (shift == 56) ? 0ull : (~0ull << (shift + 8));
+    for (int64_t i = tid; i < n_ref; i += 256) {
+      const unsigned long long k = (unsigned long long)__double_as_longlong(s[i]);
+      if ((k & mhi) == (prefix & mhi)) atomicAdd(&hist[(k >> shift) & 255ull], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned cum = 0;
+      int b = 0;
+      for (; b < 255; ++b) {
+        if (cum + hist[b] > (unsigned)kk) break;
+        cum += hist[b];
+      }
+      s_prefix = prefix | ((unsigned long long)b << shift);
+      s_kk = kk - (int)cum;
+    }
+    __syncthreads();
+    prefix = s_prefix;
+    kk = s_kk;
+  }
+  // Among keys equal to tau take the kk+1 smallest indices: radix select on
+  // the index.
+  unsigned ip = 0;
+  int ik = kk;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    const unsigned mhi = (shift == 24) ? 0u : (0xFFFFFFFFu << (shift + 8));
+    for (int64_t i = tid; i < n_ref; i += 256) {
+      if ((unsigned long long)__double_as_longlong(s[i]) != prefix) continue;
+      const unsigned k = (unsigned)i;
+      if ((k & mhi) == (ip & mhi)) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned cum = 0;
+      int b = 0;
+      for (; b < 255; ++b) {
+        if (cum + hist[b] > (unsigned)ik) break;
+        cum += hist[b];
+      }
+      s_iprefix = ip | ((unsigned)b << shift);
+      s_kk = ik - (int)cum;
+    }
+    __syncthreads();
+    ip = s_iprefix;
+    ik = s_kk;
+  }
+  if (tid == 0) s_nc = 0;
+  __syncthreads();
+  for (int64_t i = tid; i < n_ref; i += 256) {
+    const unsigned long long k = (unsigned long long)__double_as_longlong(s[i]);
+    if (k < prefix || (k == prefix && (unsigned)i <= ip)) {
+      const int pos = atomicAdd(&s_nc, 1);
+      if (pos < kCandMax) {
+        cand_idx[pos] = (int)i;
+        cand_s[pos] = s[i];
+      }
+    }
+  }
+  __syncthreads();
+  const int nc = min(s_nc, kCandMax);
+  int np2 = 64;
+  while (np2 < nc) np2 <<= 1;
+  for (int i = nc + tid; i < np2; i += 256) {
+    cand_s[i] = INFINITY;
+    cand_idx[i] = INT_MAX;
+  }
+  block_bitonic_sort(cand_s, cand_idx, np2);
+  for (int j = tid; j < k_out; j += 256) {
+    const int src = j + rank_off;
+    out_dist[q * k_out + j] = 10.0 * sqrt(fmax(cand_s[src], 0.0) / W);
+    out_idx[q * k_out + j] = cand_idx[src];
+  }
+}
+
+__global__ void init_lists_kernel(int* cnt, float* ovf, unsigned* thr, int64_t n_hdr,
+                                  int64_t n_fit) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_hdr) {
+    cnt[i] = 0;
+    ovf[i] = INFINITY;
+  }
+  if (i < n_fit) thr[i] = kInfBits;
+}
+
+__global__ void max_kernel(const double* __restrict__ v, int64_t n, double* out) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) m = fmax(m, v[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void eps_kernel(const double* __restrict__ fGt, int64_t n_fit,
+                           const double* __restrict__ gmax, double kappa, float* eps2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_fit) eps2[i] = (float)(2.0 * kappa * (fGt[i] + *gmax));
+}
+
+__global__ void identity_kernel(int64_t n_fit, int64_t n_ref, int* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_fit * n_ref) out[i] = (int)(i % n_ref);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+
+int list_capacity_for(int k1) {
+  if (k1 <= 128) return 256;
+  if (k1 <= 384) return 512;
+  if (k1 <= 896) return 1024;
+  return 0;
+}
+
+hipError_t launch_prep(const DevFrames& f, const DevWeights& w, hipStream_t st) {
+  dim3 grid(f.n_tiles), block(192);
+  switch (f.prec) {
+    case kFP32:
+      hipLaunchKernelGGL(prep_kernel<kFP32>, grid, block, 0, st, f.xyz, f.n, f.natoms,
+                         f.n_groups, w.w, w.sw, w.W, f.cen, f.G, f.Gt, f.op);
+      break;
+    case kBF16:
+      hipLaunchKernelGGL(prep_kernel<kBF16>, grid, block, 0, st, f.xyz, f.n, f.natoms,
+                         f.n_groups, w.w, w.sw, w.W, f.cen, f.G, f.Gt, f.op);
+      break;
+    case kFP16:
+      hipLaunchKernelGGL(prep_kernel<kFP16>, grid, block, 0, st, f.xyz, f.n, f.natoms,
+                         f.n_groups, w.w, w.sw, w.W, f.cen, f.G, f.Gt, f.op);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st) {
+  const int64_t n_hdr = n_fit * L.nl;
+  const int64_t n = n_hdr > n_fit ? n_hdr : n_fit;
+  hipLaunchKernelGGL(init_lists_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     L.cnt, L.ovf, L.thr, n_hdr, n_fit);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_gt(const DevFrames& f, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, st, f.Gt, (int64_t)f.n_tiles * kTile,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, float* eps2,
+                      double* gmax, hipStream_t st) {
+  hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, st, ref.Gt, (int64_t)ref.n_tiles * kTile,
+                     gmax);
+  hipLaunchKernelGGL(eps_kernel, dim3((unsigned)((fit.n + 255) / 256)), dim3(256), 0, st, fit.Gt,
+                     fit.n, gmax, kappa, eps2);
+  return hipGetLastError();
+}
+
+hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lists& L,
+                           const float* eps2, int k1, int ref_phases, bool nofit,
+                           double* dbg_scores, int grid_wgs, hipStream_t st) {
+  ContractParams P;
+  P.rop = ref.op;
+  P.fop = fit.op;
+  P.rGt = ref.Gt;
+  P.fGt = fit.Gt;
+  P.n_ref = ref.n;
+  P.n_fit = fit.n;
+  P.ref_pairs = ref.n_tiles / 2;
+  P.fit_blocks = fit.n_tiles / 2;
+  P.n_groups = ref.n_groups;
+  P.phases = ref_phases;
+  P.k1 = k1;
+  P.eps2 = eps2;
+  P.ent = L.ent;
+  P.cnt = L.cnt;
+  P.ovf = L.ovf;
+  P.thr = L.thr;
+  P.nl = L.nl;
+  P.cap = L.cap;
+  P.nofit = nofit ? 1 : 0;
+  P.dbg = dbg_scores;
+  dim3 grid(grid_wgs), block(256);
+#define MDSCTK_CONTRACT(PR, PER)                                                      \
+  hipLaunchKernelGGL((contract_kernel<PR, PER>), grid, block, 0, st, P)
+#define MDSCTK_CONTRACT_CAP(PR)             \
+  do {                                      \
+    if (L.cap == 256)                       \
+      MDSCTK_CONTRACT(PR, 4);               \
+    else if (L.cap == 512)                  \
+      MDSCTK_CONTRACT(PR, 8);               \
+    else if (L.cap == 1024)                 \
+      MDSCTK_CONTRACT(PR, 16);              \
+    else                                    \
+      return hipErrorInvalidValue;          \
+  } while (0)
+  switch (ref.prec) {
+    case kFP32: MDSCTK_CONTRACT_CAP(kFP32); break;
+    case kBF16: MDSCTK_CONTRACT_CAP(kBF16); break;
+    case kFP16: MDSCTK_CONTRACT_CAP(kFP16); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef MDSCTK_CONTRACT_CAP
+#undef MDSCTK_CONTRACT
+  return hipGetLastError();
+}
+
+hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
+                        const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
+                        bool nofit, double* out_dist, int* out_idx, int* fallback, int* n_cand,
+                        hipStream_t st) {
+  MergeParams P;
+  P.rxyz = ref.xyz;
+  P.rcen = ref.cen;
+  P.rG = ref.G;
+  P.fxyz = fit.xyz;
+  P.fcen = fit.cen;
+  P.fG = fit.G;
+  P.w = w.w;
+  P.natoms = ref.natoms;
+  P.W = w.W;
+  P.ent = L.ent;
+  P.cnt = L.cnt;
+  P.ovf = L.ovf;
+  P.nl = L.nl;
+  P.cap = L.cap;
+  P.eps2 = eps2;
+  P.k1 = k1;
+  P.k_out = k_out;
+  P.rank_off = rank_off;
+  P.nofit = nofit ? 1 : 0;
+  P.out_dist = out_dist;
+  P.out_idx = out_idx;
+  P.fallback = fallback;
+  P.n_cand = n_cand;
+  hipLaunchKernelGGL(merge_kernel, dim3((unsigned)fit.n), dim3(256), 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_exact_rows(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
+                             const int* rows, int nrows, bool nofit, double* srow,
+                             double* out_full, int64_t out_ld, hipStream_t st) {
+  ExactParams P;
+  P.rxyz = ref.xyz;
+  P.rcen = ref.cen;
+  P.rG = ref.G;
+  P.fxyz = fit.xyz;
+  P.fcen = fit.cen;
+  P.fG = fit.G;
+  P.w = w.w;
+  P.natoms = ref.natoms;
+  P.W = w.W;
+  P.n_ref = ref.n;
+  P.nofit = nofit ? 1 : 0;
+  P.out_ld = out_ld;
+  // grid.y is limited to 65535; walk the rows in chunks.
+  constexpr int kChunk = 65535;
+  for (int r0 = 0; r0 < nrows; r0 += kChunk) {
+    const int nr = nrows - r0 < kChunk ? nrows - r0 : kChunk;
+    P.rows = rows ? rows + r0 : nullptr;
+    P.row0 = rows ? 0 : r0;
+    P.srow = srow ? srow + (int64_t)r0 * ref.n : nullptr;
+    P.out_full = out_full;
+    dim3 grid((unsigned)((ref.n + kExactRefsPerWG - 1) / kExactRefsPerWG), (unsigned)nr);
+    hipLaunchKernelGGL(exact_rows_kernel, grid, dim3(256), 0, st, P);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_select_rows(const double* srow, int64_t n_ref, const int* rows, int nrows,
+                              double W, int k1, int k_out, int rank_off, double* out_dist,
+                              int* out_idx, hipStream_t st) {
+  if (k1 > kCandMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(select_rows_kernel, dim3((unsigned)nrows), dim3(256), 0, st, srow, n_ref,
+                     rows, W, k1, k_out, rank_off, out_dist, out_idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_identity_rows(int64_t n_fit, int64_t n_ref, int* out_idx, hipStream_t st) {
+  const int64_t n = n_fit * n_ref;
+  hipLaunchKernelGGL(identity_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n_fit,
+                     n_ref, out_idx);
+  return hipGetLastError();
+}
+
+}  // namespace mdsctk

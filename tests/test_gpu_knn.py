@@ -1,0 +1,106 @@
+"""GPU parity of the fused knn_rms engine against the FP64 oracle.
+
+All calls go through the C ABI (libmdsctk_knn.so via ctypes)."""
+import numpy as np
+import pytest
+
+import oracle
+from mdsctk_amd import knn, synth
+from tests.parity import check_knn, TOL_A
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ["fp32", "bf16", "fp16", "exact"]
+
+
+def _case(n_ref, n_fit, natoms, seed, wkind="uniform"):
+    ref = synth.trajectory(n_ref, natoms, seed)
+    fit = synth.trajectory(n_fit, natoms, seed + 1000, base_seed=seed)
+    w = synth.masses(natoms, wkind, seed)
+    return ref, fit, w
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("nofit", [False, True])
+def test_approx_scores_within_window(prec, nofit):
+    """The contraction's approximate score stays inside the stated window."""
+    ref, fit, w = _case(96, 80, 57, 11, "backbone")
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(ref, w)
+        a, e = ctx.debug_scores(fit, flags=knn.KNN_NOFIT if nofit else 0)
+        G = np.array([oracle.center(x, w)[0] for x in ref])
+        Gf = np.array([oracle.center(x, w)[0] for x in fit])
+    # exact scores agree with the oracle's FP64 value
+    S = e / w.astype(np.float64).sum()
+    d_gpu = np.sqrt(np.maximum(S, 0))
+    d_orc = oracle.full_exact(ref, fit, w, nofit=nofit) / 10.0
+    assert np.abs(d_gpu - d_orc).max() < 1e-9
+    rel = np.abs(a - e) / (Gf[:, None] + G.max())
+    kappa = {"fp32": 2e-6, "bf16": 2e-5, "fp16": 2e-6}[prec]
+    print(prec, nofit, "max rel", rel.max())
+    assert rel.max() < kappa / 4
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_knn_small_vs_oracle(prec):
+    ref, fit, w = _case(300, 170, 37, 3, "random")
+    k = 12
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(ref, w)
+        d, i = ctx.query(fit, k)
+        st = ctx.stats()
+    d_o, i_o = oracle.knn_exact(ref, fit, w, k)
+    check_knn(ref, fit, w, d, i, d_o, i_o)
+    assert st["n_fallback_rows"] == (fit.shape[0] if prec == "exact" else 0)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_knn_self_query(prec):
+    ref, _, w = _case(257, 1, 61, 5)
+    k = 9
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(ref, w)
+        d, i = ctx.query(None, k)
+    d_o, i_o = oracle.knn_exact(ref, ref, w, k)
+    check_knn(ref, ref, w, d, i, d_o, i_o)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "exact"])
+def test_full_rows_and_nofit(prec):
+    ref, fit, w = _case(70, 33, 25, 7, "backbone")
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(ref, w)
+        d, i = ctx.query(fit, 0, knn.KNN_FULL_ROWS)
+        dn, iN = ctx.query(fit, 5, knn.KNN_NOFIT)
+    full = oracle.full_exact(ref, fit, w)
+    assert np.abs(d - full).max() < TOL_A
+    assert (i == np.arange(ref.shape[0])[None, :]).all()
+    d_o, i_o = oracle.knn_exact(ref, fit, w, 5, nofit=True)
+    check_knn(ref, fit, w, dn, iN, d_o, i_o, nofit=True)
+
+
+def test_keep_rank0_and_k_clamp():
+    ref, fit, w = _case(20, 9, 16, 9)
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_reference(ref, w)
+        d, i = ctx.query(fit, 50)          # k clamps to n_ref-1
+        assert d.shape == (9, 19)
+        d0, i0 = ctx.query(fit, 4, knn.KNN_KEEP_RANK0)
+    d_o, i_o = oracle.knn_exact(ref, fit, w, 50)
+    check_knn(ref, fit, w, d, i, d_o, i_o)
+    d_o0, i_o0 = oracle.knn_exact(ref, fit, w, 4, keep_rank0=True)
+    check_knn(ref, fit, w, d0, i0, d_o0, i_o0)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_c1_workload(prec):
+    """2,000 x 2,000 frames x 304 atoms, k=30, self query (the reference's CPU case)."""
+    ref = synth.trajectory(2000, 304, 1)
+    w = synth.masses(304, "uniform")
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(ref, w)
+        d, i = ctx.query(None, 30)
+        st = ctx.stats()
+    d_o, i_o = oracle.knn_exact(ref, ref, w, 30)
+    err, same = check_knn(ref, ref, w, d, i, d_o, i_o)
+    print(prec, "C1 max err A", err, "same-index fraction", same, st)
