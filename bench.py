@@ -1,0 +1,239 @@
+"""knn_rms on one MI355X: all-pairs fitted RMSD + per-query top-k.
+
+Default workload (the precision-form configuration): 200,000 reference x
+200,000 query frames (self query) x 1,000 atoms, k = 64, split-BF16
+contraction with FP64 G terms, QCP and FP64 re-score.  Frames come from the
+seeded metastable generator (50 centres, 0.1 nm noise, 5% near-duplicates at
+1e-4 nm).  One step = one full query of the resident set (contraction, QCP,
+top-k, FP64 re-score, outputs written to device memory).
+
+Beside the headline number it reports, on the same data, the plain-FP32
+contraction path, a serving-latency probe (1,000-frame queries against a
+100,000 x 500 resident reference, k = 50), the contraction kernel's roofline
+(HIP events on the engine's stream), accuracy against the FP64 oracle on a
+sample, and the reference-equivalent CPU knn_rms timed on a bounded sample.
+
+Prints ONE JSON line.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RMSD pairs/sec: plain FP32 vs split-BF16 MFMA at <=1e-5 nm; p99 query latency"
+PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp32": 157.3}
+FLOPS_PER_ATOM = {"bf16": 54, "fp16": 54, "fp32": 18}
+
+
+class Hip:
+    """Minimal HIP runtime access for device buffers (no torch needed)."""
+
+    def __init__(self):
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                self.L = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        else:
+            raise RuntimeError("libamdhip64.so not found")
+        self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.L.hipFree.argtypes = [ctypes.c_void_p]
+        self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.L.hipDeviceSynchronize.argtypes = []
+
+    def malloc(self, n):
+        p = ctypes.c_void_p()
+        rc = self.L.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc({n}) failed: {rc}")
+        return p.value
+
+    def free(self, p):
+        self.L.hipFree(ctypes.c_void_p(p))
+
+    def d2h(self, host, dev, n):
+        rc = self.L.hipMemcpy(host.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(dev),
+                              ctypes.c_size_t(n), 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+
+    def sync(self):
+        self.L.hipDeviceSynchronize()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_fused(ctx, hip, n_fit, k, steps, warmup):
+    d_dist = hip.malloc(n_fit * k * 8)
+    d_idx = hip.malloc(n_fit * k * 4)
+    try:
+        for _ in range(warmup):
+            ctx.query_device(0, n_fit, k, 0, d_dist, d_idx)
+        hip.sync()
+        stats = []
+        t0 = time.perf_counter()
+        for s in range(steps):
+            ctx.query_device(0, n_fit, k, 0, d_dist, d_idx)
+            stats.append(ctx.stats())
+            log(f"  step {s}: {stats[-1]['ms_total']:.1f} ms device, contraction "
+                f"{stats[-1]['ms_contract']:.1f} ms, merge {stats[-1]['ms_merge']:.1f} ms, "
+                f"fallback rows {stats[-1]['n_fallback_rows']}")
+        hip.sync()
+        wall = (time.perf_counter() - t0) / steps
+        dist = np.zeros((n_fit, k), np.float64)
+        idx = np.zeros((n_fit, k), np.int32)
+        hip.d2h(dist, d_dist, dist.nbytes)
+        hip.d2h(idx, d_idx, idx.nbytes)
+    finally:
+        hip.free(d_dist)
+        hip.free(d_idx)
+    return wall, stats, dist, idx
+
+
+def mean_stat(stats, key):
+    return float(np.mean([s[key] for s in stats]))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-frames", type=int, default=200_000)
+    ap.add_argument("--natoms", type=int, default=1000)
+    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16", "fp32"])
+    ap.add_argument("--fp32-steps", type=int, default=1, help="steps of the FP32 comparator (0 = skip)")
+    ap.add_argument("--serving-queries", type=int, default=20, help="1k-frame latency probes (0 = skip)")
+    ap.add_argument("--cpu-rows", type=int, default=16, help="fit rows of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--check-rows", type=int, default=8, help="rows checked against the FP64 oracle")
+    ap.add_argument("--seed", type=int, default=2026)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the knn_rms engine runs on a single "
+              "MI355X; see DESIGN.md for how the path would shard). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from mdsctk_amd import knn, synth
+
+    hip = Hip()
+    n, natoms, k, prec = args.n_frames, args.natoms, args.k, args.precision
+    t = time.time()
+    frames = synth.metastable_trajectory(n, natoms, args.seed)
+    w = synth.masses(natoms, "uniform")
+    log(f"generated {n} x {natoms} frames in {time.time() - t:.1f} s")
+
+    out = {"metric": METRIC, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": prec, "data": "synthetic"}
+    out["config"] = {"workload": f"self kNN {n} x {n} frames x {natoms} atoms, k={k}, "
+                                 f"metastable generator (50 centres, 5% near-duplicates)",
+                     "n_ref": n, "n_fit": n, "natoms": natoms, "k": k,
+                     "precision": {"bf16": "split-BF16 hi/lo, 3 MFMA products, FP32 accumulate",
+                                   "fp16": "split-FP16 hi/lo, 3 MFMA products, FP32 accumulate",
+                                   "fp32": "FP32 MFMA"}[prec],
+                     "epilogue": "FP64 G terms + FP64 QCP Newton, FP64 re-score of the candidate window"}
+
+    pairs = float(n) * n
+    with knn.KnnContext(0, prec) as ctx:
+        t = time.time()
+        ctx.set_reference(frames, w)
+        log(f"reference resident in HBM ({time.time() - t:.1f} s)")
+        wall, stats, dist, idx = time_fused(ctx, hip, n, k, args.steps, args.warmup)
+    ms_contract = mean_stat(stats, "ms_contract")
+    out["value"] = pairs / wall
+    out["ms_per_step"] = wall * 1e3
+    flops = FLOPS_PER_ATOM[prec] * natoms * pairs
+    achieved = flops / (ms_contract * 1e-3) / 1e12
+    out["roofline"] = {"bound": "mfma", "kernel": "contract_kernel",
+                       "achieved": achieved, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
+                       "frac": achieved / PEAK_TFLOPS[prec], "traffic": None,
+                       "flops_per_launch": flops, "ms_per_launch": ms_contract,
+                       "flops_per_pair": FLOPS_PER_ATOM[prec] * natoms}
+    out["stages_ms"] = {s: mean_stat(stats, "ms_" + s) for s in ("prep", "contract", "merge", "fallback", "total")}
+    out["candidates_per_query"] = mean_stat(stats, "n_candidates") / n
+    out["fallback_rows"] = int(max(s["n_fallback_rows"] for s in stats))
+
+    # accuracy vs the FP64 oracle: 10k output pairs and a few full rows
+    import oracle
+    rng = np.random.default_rng(args.seed)
+    rows = rng.integers(0, n, 10000)
+    cols = rng.integers(0, k, 10000)
+    ref_idx = idx[rows, cols].astype(np.int64)
+    exact = oracle.pairs_nm(frames, frames, w, ref_idx, rows.astype(np.int64))
+    err_nm = float(np.abs(dist[rows, cols] / 10.0 - exact).max())
+    chk = rng.choice(n, size=args.check_rows, replace=False) if args.check_rows else np.zeros(0, int)
+    match = 1.0
+    if len(chk):
+        t = time.time()
+        d_o, i_o = oracle.knn_exact(frames, frames[chk], w, k)
+        dd = np.abs(dist[chk] - d_o).max()
+        sets = np.mean([len(set(idx[r]) & set(i_o[j])) / k for j, r in enumerate(chk)])
+        match = float(sets)
+        err_nm = max(err_nm, float(dd) / 10.0)
+        log(f"oracle rows checked in {time.time() - t:.1f} s")
+    out["accuracy"] = {"max_err_nm": err_nm, "tolerance_nm": 1e-5, "pairs_checked": 10000,
+                       "rows_checked": int(len(chk)), "index_set_match": match}
+
+    if args.fp32_steps > 0 and prec != "fp32":
+        with knn.KnnContext(0, "fp32") as ctx:
+            ctx.set_reference(frames, w)
+            wall32, st32, _, _ = time_fused(ctx, hip, n, k, args.fp32_steps, 0)
+        ms32 = mean_stat(st32, "ms_contract")
+        a32 = 18 * natoms * pairs / (ms32 * 1e-3) / 1e12
+        out["fp32_path"] = {"value": pairs / wall32, "unit": "pairs/s", "ms_per_step": wall32 * 1e3,
+                            "contract_ms": ms32, "achieved_tflops": a32, "frac_fp32_peak": a32 / 157.3}
+        out["speedup_vs_fp32"] = out["value"] / out["fp32_path"]["value"]
+
+    if args.serving_queries > 0:
+        n_res, na_res, q_frames, kq = 100_000, 500, 1000, 50
+        res = synth.trajectory(n_res, na_res, args.seed + 7)
+        qs = synth.trajectory(args.serving_queries * q_frames, na_res, args.seed + 8,
+                              base_seed=args.seed + 7)
+        lat = []
+        with knn.KnnContext(0, prec) as ctx:
+            ctx.set_reference(res, synth.masses(na_res))
+            ctx.query(qs[:q_frames], kq)  # warm-up
+            for j in range(args.serving_queries):
+                t = time.perf_counter()
+                ctx.query(qs[j * q_frames:(j + 1) * q_frames], kq)
+                lat.append((time.perf_counter() - t) * 1e3)
+        lat = np.array(lat)
+        out["serving"] = {"reference": f"{n_res} x {na_res} atoms resident", "query_frames": q_frames,
+                          "k": kq, "requests": len(lat), "p50_ms": float(np.percentile(lat, 50)),
+                          "p99_ms": float(np.percentile(lat, 99)),
+                          "pairs_per_s": q_frames * n_res / (np.mean(lat) * 1e-3),
+                          "mode": "sequential requests, host buffers in and out (PCIe included)"}
+
+    if args.cpu_rows > 0:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t = time.time()
+        cref = oracle.port_center(frames, w)
+        log(f"CPU centring {time.time() - t:.1f} s")
+        t = time.perf_counter()
+        oracle.port_knn_rows(cref, cref, w, k, 0, args.cpu_rows, nthreads=threads)
+        cpu_s = time.perf_counter() - t
+        cpu_pairs = args.cpu_rows * float(n)
+        out["cpu_baseline"] = {"value": cpu_pairs / cpu_s, "unit": "pairs/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"first {args.cpu_rows} fit rows x all {n} refs "
+                                         f"({cpu_pairs:.3g} pairs, {cpu_s:.1f} s), reference-"
+                                         "equivalent float knn_rms (OpenMP over fit rows)"}
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

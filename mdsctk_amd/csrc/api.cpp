@@ -19,15 +19,19 @@ using namespace mdsctk;
 namespace {
 
 // Candidate window: |s~ - s| <= kappa (G_q + max G_ref) for the approximate
-// score s~ of each contraction form, with s = G_q + G_r - 2 lambda.  The
-// constants are a safety factor over the largest relative deviation
-// measured on the synthetic generators (see the calibration test); they are
-// not worst-case proofs.
-double kappa_for(int prec) {
+// score s~ of each contraction form (s = G_q + G_r - 2 lambda).  kappa is an
+// envelope a*sqrt(N) + b fitted to the largest relative deviation measured
+// on the synthetic generators (random, metastable with near-duplicates, and
+// pre-aligned frames) for N = 20..2000 atoms, times a safety factor of 2.
+// FP32 accumulation contributes the sqrt(N) term (one rounding per MFMA
+// step); the split forms add a constant term from the dropped lo*lo'
+// product.  It is a measured statistical envelope, not a worst-case proof.
+double kappa_for(int prec, int natoms) {
+  const double rn = std::sqrt((double)std::max(natoms, 16));
   switch (prec) {
-    case kFP32: return 2.0e-6;
-    case kBF16: return 2.0e-5;
-    case kFP16: return 2.0e-6;
+    case kFP32: return 2.0 * (5.5e-8 * rn);
+    case kBF16: return 2.0 * (3.0e-8 * rn + 4.0e-6);
+    case kFP16: return 2.0 * (3.0e-8 * rn + 1.0e-7);
     default: return 0.0;
   }
 }
@@ -143,7 +147,7 @@ void free_frames(mdsctk_knn_ctx* c, DevFrames& f) {
 }
 
 size_t operand_bytes(int prec, int n_tiles, int n_groups) {
-  return (size_t)n_tiles * n_groups * blocks_per_group(prec) * 64 * 16;
+  return ((size_t)n_tiles * n_groups + kOperandTailGroups) * blocks_per_group(prec) * 64 * 16;
 }
 
 // Allocate the derived arrays of a frame set (xyz handled by the caller).
@@ -152,8 +156,7 @@ int alloc_frames(mdsctk_knn_ctx* c, DevFrames& f, int64_t n, int natoms, int pre
   f.natoms = natoms;
   f.prec = prec == kExact ? kFP32 : prec;
   const int64_t tiles = (n + kTile - 1) / kTile;
-  f.n_tiles = (int)((tiles + 1) & ~int64_t(1));
-  if (f.n_tiles < 2) f.n_tiles = 2;
+  f.n_tiles = (int)((tiles + kTileAlign - 1) / kTileAlign * kTileAlign);
   const int ga = group_atoms(f.prec);
   f.n_groups = (natoms + ga - 1) / ga;
   const size_t rows = (size_t)f.n_tiles * kTile;
@@ -363,8 +366,11 @@ int query_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int64_t n_fi
   } else {
     // Fused path.
     const int cap = list_capacity_for(k1);
-    const int fit_blocks = fit->n_tiles / 2, ref_pairs = c->ref.n_tiles / 2;
-    const int grid = std::max(8, (2 * c->n_cu) & ~7);
+    const int ft = contract_fit_tiles(fit->prec);
+    const int fit_blocks = fit->n_tiles / ft;
+    const int ref_pairs = c->ref.n_tiles / kRefTilesPerWG;
+    // workgroups resident at once: two 4-wave or one 8-wave per CU
+    const int grid = std::max(8, (c->n_cu * (ft == 2 ? 2 : 1)) & ~7);
     int phases = c->forced_phases;
     if (phases <= 0) {
       phases = std::max(8, (grid + fit_blocks - 1) / fit_blocks);
@@ -391,7 +397,7 @@ int query_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int64_t n_fi
     L.thr = (unsigned*)sp; sp += n_fit * sizeof(unsigned);
     float* eps2 = (float*)sp; sp += n_fit * sizeof(float);
     int* d_fb = (int*)sp;
-    const double kappa = kappa_for(c->prec) * c->eps_scale;
+    const double kappa = kappa_for(c->prec, c->ref.natoms) * c->eps_scale;
     HIP_TRY(c, launch_init_lists(L, n_fit, c->st));
     HIP_TRY(c, launch_eps(*fit, c->ref, kappa, eps2, c->gmax, c->st));
     HIP_TRY(c, hipMemsetAsync(c->n_cand, 0, sizeof(int), c->st));
@@ -555,7 +561,7 @@ int mdsctk_knn_debug_scores(mdsctk_knn_ctx* c, const float* xyz_nm, int64_t n_fi
   L.nl = 4;
   L.cap = 256;
   hipError_t e = launch_contract(c->ref, *fit, L, nullptr, 1, 1, nofit, (double*)buf,
-                                 std::max(8, (2 * c->n_cu) & ~7), c->st);
+                                 std::max(8, c->n_cu & ~7), c->st);  // any grid is valid
   if (e == hipSuccess)
     e = hipMemcpyAsync(approx_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, c->st);
   if (e == hipSuccess && exact_s) {

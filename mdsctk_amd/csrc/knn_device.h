@@ -10,6 +10,11 @@
 namespace mdsctk {
 
 constexpr int kTile = 32;  // frames per MFMA tile edge (v_mfma_*_32x32xK)
+// Contraction workgroup: one wave per 32x32 (ref x fit) frame tile,
+// 2 reference tiles x contract_fit_tiles(prec) fit tiles.
+constexpr int kRefTilesPerWG = 2;
+// Frame sets are padded to a multiple of this many tiles.
+constexpr int kTileAlign = 4;
 
 enum Prec : int { kFP32 = 0, kBF16 = 1, kFP16 = 2, kExact = 3 };
 
@@ -18,13 +23,17 @@ enum Prec : int { kFP32 = 0, kBF16 = 1, kFP16 = 2, kExact = 3 };
 inline int group_atoms(int prec) { return prec == kFP32 ? 8 : 16; }
 // 16-byte operand blocks of 64 lanes per (tile, K-group): 3 planes x parts.
 inline int blocks_per_group(int prec) { return prec == kFP32 ? 3 : 6; }
+// K-groups of zero slack after a frame set's operand: the contraction's
+// copy pipeline reads up to four groups past the end of the last tile.
+constexpr int kOperandTailGroups = 4;
 
 // A frame set resident in HBM.  Row counts are padded to a multiple of
-// 2*kTile (the contraction walks tiles in pairs); padded rows are zero.
+// kTileAlign*kTile (the contraction walks reference tiles in pairs and fit
+// tiles in fours); padded rows are zero.
 struct DevFrames {
   int64_t n = 0;        // real frames
   int natoms = 0;
-  int n_tiles = 0;      // padded tile count (even)
+  int n_tiles = 0;      // padded tile count (multiple of kTileAlign)
   int n_groups = 0;     // operand K-groups
   int prec = kFP32;
   float* xyz = nullptr;   // [n][natoms][3] as given (nm)
@@ -82,6 +91,8 @@ hipError_t launch_select_rows(const double* srow, int64_t n_ref, const int* rows
 hipError_t launch_identity_rows(int64_t n_fit, int64_t n_ref, int* out_idx, hipStream_t st);
 hipError_t launch_max_gt(const DevFrames& f, double* out, hipStream_t st);
 
+// Fit tiles per contraction workgroup for a precision (2 or 4).
+int contract_fit_tiles(int prec);
 // Choose the per-list capacity for k1 (0 when the fused path cannot serve it).
 int list_capacity_for(int k1);
 

@@ -162,18 +162,20 @@ __device__ double qcp_lambda_exact(const double* S_in, double ga, double gb) {
   for (int i = 0; i < 9; ++i) S[i] = S_in[i] / g;
   double c2, c1, c0;
   qcp_coeffs(S, c2, c1, c0);
-  double l = 1.0;
+  double l = 1.0, dp = 0.0;
   bool conv = false;
   for (int it = 0; it < 100; ++it) {
     const double l2 = l * l;
     const double p = (l2 + c2) * l2 + c1 * l + c0;
-    const double dp = (4.0 * l2 + 2.0 * c2) * l + c1;
+    dp = (4.0 * l2 + 2.0 * c2) * l + c1;
     if (dp == 0.0) break;
     const double step = p / dp;
     l -= step;
     if (fabs(step) <= 1e-15 * fabs(l)) { conv = true; break; }
   }
-  if (!conv) l = jacobi4_max(S);
+  // A nearly double largest root (near-collinear frames) leaves Newton with
+  // about half the digits; the Jacobi eigen-solve resolves it.
+  if (!conv || fabs(dp) < 1e-4) l = jacobi4_max(S);
   return l * g;
 }
 
@@ -374,7 +376,10 @@ __global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz
           lo[j] = f16_rne((float)(v - vh));
           vl = f16_val(lo[j]);
         }
-        gop += vh * vh + 2.0 * vh * vl;
+        // G~ of the represented value (hi+lo)^2: unbiased against the three-
+        // product cross term, whose dropped lo*lo' averages out between
+        // frames that are not atom-by-atom aligned.
+        gop += (vh + vl) * (vh + vl);
       }
       uint4 oh, ol;
       oh.x = hi[0] | ((unsigned)hi[1] << 16); oh.y = hi[2] | ((unsigned)hi[3] << 16);
@@ -417,7 +422,7 @@ struct ContractParams {
   const double* rGt;
   const double* fGt;
   int64_t n_ref, n_fit;
-  int ref_pairs, fit_blocks, n_groups, phases, k1;
+  int ref_pairs, fit_tiles, n_groups, phases, k1;
   const float* eps2;
   uint2* ent;
   int* cnt;
@@ -563,35 +568,75 @@ __device__ __attribute__((noinline)) void compact_lists(uint2* __restrict__ ent,
   }
 }
 
-template <int PREC, int PER>
-__global__ __launch_bounds__(256, 2) void contract_kernel(ContractParams P) {
+// End of one pipeline step: this wave's copies of the stage to be read next
+// have landed (at most N newer copies -- the stage just issued -- stay in
+// flight), its LDS reads are done, then the workgroup barrier.  The empty
+// asm statements keep the compiler from moving memory accesses across the
+// barrier (s_barrier itself is not a memory fence).
+template <int N>
+__device__ __forceinline__ void stage_barrier() {
+  static_assert(N >= 0 && N < 16, "vmcnt range");
+  asm volatile("" ::: "memory");
+  // s_waitcnt: vmcnt = N, expcnt = 7 (no wait), lgkmcnt = 0
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4));
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// FT = fit tiles per workgroup (2 or 4); the workgroup has 2*FT waves, one
+// per (reference tile, fit tile) pair.
+template <int PREC, int PER, int FT>
+__global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(ContractParams P) {
   constexpr int BPG = (PREC == kFP32) ? 3 : 6;
   constexpr int TILE_VEC = BPG * 64;
-  constexpr int STAGE_VEC = 4 * TILE_VEC;
-  // Two stage buffers as distinct LDS objects, so that the compiler's
-  // LDS-DMA dependency tracking lets reads of one overlap copies into the
-  // other.  Stage layout: [ref tile 0][ref tile 1][fit tile 0][fit tile 1].
+  constexpr int STAGE_VEC = (kRefTilesPerWG + FT) * TILE_VEC;
+  // Three stage buffers (two K-groups in flight while one is consumed),
+  // declared as distinct LDS objects so that the compiler does not serialise
+  // reads of one behind copies into another.  Stage layout:
+  // [ref tile 0][ref tile 1][fit tile 0] ... [fit tile FT-1].
   __shared__ uint4 sbuf0[STAGE_VEC];
   __shared__ uint4 sbuf1[STAGE_VEC];
-  __shared__ double sGr[2 * kTile];
+  __shared__ uint4 sbuf2[STAGE_VEC];
+  __shared__ double sGr[2][2 * kTile];  // G~ of the current / next reference tile pair
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  const int ri = wave & 1;
-  const int fi = wave >> 1;
+  const int ri = wave & 1;   // reference tile of this wave
+  const int fi = wave >> 1;  // fit tile of this wave
+  const int ng = P.n_groups;
+  // Stages per tile, a multiple of three so that the buffer rotation lines
+  // up with tile boundaries; stages g >= ng copy padding and skip the MFMAs.
+  const int ng3 = (ng + 2) / 3 * 3;
+
+  // Waves 0..kSlots-1 each copy one stage slot (slot = wave); the others
+  // only compute.
+  constexpr int kSlots = kRefTilesPerWG + FT;
+  const bool copier = wave < kSlots;
+  uint4* const d0 = sbuf0 + (copier ? wave : 0) * TILE_VEC;
+  uint4* const d1 = sbuf1 + (copier ? wave : 0) * TILE_VEC;
+  uint4* const d2 = sbuf2 + (copier ? wave : 0) * TILE_VEC;
+  const uint4* const a0 = sbuf0 + ri * TILE_VEC;
+  const uint4* const a1 = sbuf1 + ri * TILE_VEC;
+  const uint4* const a2 = sbuf2 + ri * TILE_VEC;
+  const uint4* const b0 = sbuf0 + (kRefTilesPerWG + fi) * TILE_VEC;
+  const uint4* const b1 = sbuf1 + (kRefTilesPerWG + fi) * TILE_VEC;
+  const uint4* const b2 = sbuf2 + (kRefTilesPerWG + fi) * TILE_VEC;
+  const int fit_blocks = P.fit_tiles / FT;
 
   // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
   // that consecutive work items (same fit block, neighbouring reference
   // phases) land on the same XCD and share its L2.
   const int nwg = gridDim.x;
   const int logical = (int)(blockIdx.x & 7) * (nwg >> 3) + (int)(blockIdx.x >> 3);
-  const int n_items = P.fit_blocks * P.phases;
+  const int n_items = fit_blocks * P.phases;
 
   for (int item = logical; item < n_items; item += nwg) {
     const int fb = item / P.phases;
     const int ph = item - fb * P.phases;
-    const int64_t fit_idx = (int64_t)(2 * fb + fi) * kTile + (lane & 31);
+    const int n_tiles = (P.ref_pairs - ph + P.phases - 1) / P.phases;  // ref pairs ph, ph+R, ...
+    if (n_tiles <= 0) continue;
+    const int64_t fit_idx = (int64_t)(FT * fb + fi) * kTile + (lane & 31);
     const bool fvalid = fit_idx < P.n_fit;
     const int list_id = (ph * 2 + ri) * 2 + (lane >> 5);
     const int64_t hdr = fit_idx * P.nl + list_id;
@@ -601,36 +646,68 @@ __global__ __launch_bounds__(256, 2) void contract_kernel(ContractParams P) {
     int cnt = 0;
     float thr = INFINITY, ovf = INFINITY;
 
-    for (int p = ph; p < P.ref_pairs; p += P.phases) {
-      if (fvalid && P.thr) thr = fminf(thr, __uint_as_float(__builtin_nontemporal_load(&P.thr[fit_idx])));
+    // Stage s of this item is K-group g = s % ng3 of tile t = s / ng3.  Wave
+    // w < 2 copies ref tile 2p+w (p = ph + t*R), wave 2..FT+1 fit tile
+    // FT*fb+w-2.  Stages past the item's last tile re-read its last tile
+    // (never used).
+    const uint4* const fit_base =
+        P.fop + (int64_t)(FT * fb + (copier ? wave : 2) - kRefTilesPerWG) * ng * TILE_VEC;
+    auto src_of = [&](int t, int g) -> const uint4* {
+      if (t >= n_tiles) t = n_tiles - 1;
+      if (wave >= kRefTilesPerWG) return fit_base + (int64_t)g * TILE_VEC;
+      const int p = ph + t * P.phases;
+      return P.rop + ((int64_t)(kRefTilesPerWG * p + wave) * ng + g) * TILE_VEC;
+    };
+    auto dma_stage = [&](int t, int g, uint4* dst) {
+      if (!copier) return;
+      if (g >= ng3) { g -= ng3; ++t; }
+      const uint4* src = src_of(t, g) + lane;
+#pragma unroll
+      for (int j = 0; j < BPG; ++j)
+        __builtin_amdgcn_global_load_lds((const void*)(src + j * 64),
+                                         (lds_void_t*)(dst + j * 64), 16, 0, 0);
+    };
+
+    __syncthreads();  // previous item's readers of sbuf*/sGr are done
+    if (tid < 2 * kTile) sGr[0][tid] = P.rGt[(int64_t)ph * 2 * kTile + tid];
+    dma_stage(0, 0, d0);
+    dma_stage(0, 1, d1);
+    stage_barrier<BPG>();
+
+    for (int t = 0; t < n_tiles; ++t) {
+      const int p = ph + t * P.phases;
+      // Loads consumed only at the end of the tile: the next tile's G~ and
+      // the shared per-query threshold.
+      double g_next = 0.0;
+      if (tid < 2 * kTile && t + 1 < n_tiles)
+        g_next = P.rGt[(int64_t)(p + P.phases) * 2 * kTile + tid];
+      unsigned thr_shared = kInfBits;
+      if (fvalid && P.thr) thr_shared = __builtin_nontemporal_load(&P.thr[fit_idx]);
+
       f32x16 acc[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) acc[q] = (f32x16){0};
-
-      // Wave w copies stage slot w: ref tiles 2p, 2p+1, fit tiles 2fb, 2fb+1.
-      const uint4* src_tile =
-          (wave < 2) ? P.rop + (int64_t)(2 * p + wave) * P.n_groups * TILE_VEC
-                     : P.fop + (int64_t)(2 * fb + wave - 2) * P.n_groups * TILE_VEC;
-      __syncthreads();
-      if (tid < 2 * kTile) sGr[tid] = P.rGt[(int64_t)p * 2 * kTile + tid];
-      dma_group<BPG>(src_tile, 0, sbuf0 + wave * TILE_VEC, lane);
-      __syncthreads();
-      int g = 0;
-      for (; g + 1 < P.n_groups; g += 2) {
-        dma_group<BPG>(src_tile, g + 1, sbuf1 + wave * TILE_VEC, lane);
-        mma_group<PREC>(acc, sbuf0 + ri * TILE_VEC, sbuf0 + (2 + fi) * TILE_VEC, lane);
-        __syncthreads();
-        if (g + 2 < P.n_groups) dma_group<BPG>(src_tile, g + 2, sbuf0 + wave * TILE_VEC, lane);
-        mma_group<PREC>(acc, sbuf1 + ri * TILE_VEC, sbuf1 + (2 + fi) * TILE_VEC, lane);
-        __syncthreads();
+      // Stage g lives in buffer g % 3; while it is consumed, stage g+2 (in
+      // this tile or the next) is being copied.
+      for (int g = 0; g < ng3; g += 3) {
+        dma_stage(t, g + 2, d2);
+        mma_group<PREC>(acc, a0, b0, lane);
+        stage_barrier<BPG>();
+        dma_stage(t, g + 3, d0);
+        if (g + 1 < ng) mma_group<PREC>(acc, a1, b1, lane);
+        stage_barrier<BPG>();
+        dma_stage(t, g + 4, d1);
+        if (g + 2 < ng) mma_group<PREC>(acc, a2, b2, lane);
+        stage_barrier<BPG>();
       }
-      if (g < P.n_groups) {
-        mma_group<PREC>(acc, sbuf0 + ri * TILE_VEC, sbuf0 + (2 + fi) * TILE_VEC, lane);
-        __syncthreads();
-      }
+      // Publish the next tile's G~ (read by its epilogue, after at least one
+      // more barrier).
+      if (tid < 2 * kTile && t + 1 < n_tiles) sGr[(t + 1) & 1][tid] = g_next;
+      if (fvalid && P.thr) thr = fminf(thr, __uint_as_float(thr_shared));
 
       // Epilogue: lane holds fit frame (lane%32) against 16 ref frames.
-      const int64_t ref_base = (int64_t)(2 * p + ri) * kTile;
+      const double* const gr_tile = sGr[t & 1];
+      const int64_t ref_base = (int64_t)(kRefTilesPerWG * p + ri) * kTile;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int rl = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -639,7 +716,7 @@ __global__ __launch_bounds__(256, 2) void contract_kernel(ContractParams P) {
         float m[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) m[q] = acc[q][r];
-        const double gr = sGr[ri * kTile + rl];
+        const double gr = gr_tile[ri * kTile + rl];
         if (P.dbg) {
           P.dbg[fit_idx * P.n_ref + ref_idx] = qcp_score_approx(m, gr, gf, INFINITY, P.nofit);
           continue;
@@ -1025,6 +1102,8 @@ __global__ void identity_kernel(int64_t n_fit, int64_t n_ref, int* out) {
 // ---------------------------------------------------------------------------
 // launchers
 
+int contract_fit_tiles(int prec) { return prec == kFP32 ? 2 : 4; }
+
 int list_capacity_for(int k1) {
   if (k1 <= 128) return 256;
   if (k1 <= 384) return 512;
@@ -1086,8 +1165,8 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   P.fGt = fit.Gt;
   P.n_ref = ref.n;
   P.n_fit = fit.n;
-  P.ref_pairs = ref.n_tiles / 2;
-  P.fit_blocks = fit.n_tiles / 2;
+  P.ref_pairs = ref.n_tiles / kRefTilesPerWG;
+  P.fit_tiles = fit.n_tiles;
   P.n_groups = ref.n_groups;
   P.phases = ref_phases;
   P.k1 = k1;
@@ -1100,24 +1179,27 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   P.cap = L.cap;
   P.nofit = nofit ? 1 : 0;
   P.dbg = dbg_scores;
-  dim3 grid(grid_wgs), block(256);
-#define MDSCTK_CONTRACT(PR, PER)                                                      \
-  hipLaunchKernelGGL((contract_kernel<PR, PER>), grid, block, 0, st, P)
-#define MDSCTK_CONTRACT_CAP(PR)             \
+  dim3 grid(grid_wgs);
+  // FP32 runs two 4-wave workgroups per CU (2 x 2 tiles each); the split
+  // forms one 8-wave workgroup per CU (2 x 4 tiles: a third less L2 traffic
+  // per MFMA).
+#define MDSCTK_CONTRACT(PR, PER, FT) \
+  hipLaunchKernelGGL((contract_kernel<PR, PER, FT>), grid, dim3(64 * kRefTilesPerWG * FT), 0, st, P)
+#define MDSCTK_CONTRACT_CAP(PR, FT)         \
   do {                                      \
     if (L.cap == 256)                       \
-      MDSCTK_CONTRACT(PR, 4);               \
+      MDSCTK_CONTRACT(PR, 4, FT);           \
     else if (L.cap == 512)                  \
-      MDSCTK_CONTRACT(PR, 8);               \
+      MDSCTK_CONTRACT(PR, 8, FT);           \
     else if (L.cap == 1024)                 \
-      MDSCTK_CONTRACT(PR, 16);              \
+      MDSCTK_CONTRACT(PR, 16, FT);          \
     else                                    \
       return hipErrorInvalidValue;          \
   } while (0)
   switch (ref.prec) {
-    case kFP32: MDSCTK_CONTRACT_CAP(kFP32); break;
-    case kBF16: MDSCTK_CONTRACT_CAP(kBF16); break;
-    case kFP16: MDSCTK_CONTRACT_CAP(kFP16); break;
+    case kFP32: MDSCTK_CONTRACT_CAP(kFP32, 2); break;
+    case kBF16: MDSCTK_CONTRACT_CAP(kBF16, 4); break;
+    case kFP16: MDSCTK_CONTRACT_CAP(kFP16, 4); break;
     default: return hipErrorInvalidValue;
   }
 #undef MDSCTK_CONTRACT_CAP

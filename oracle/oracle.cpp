@@ -145,15 +145,20 @@ double qcp_lambda(const double* S_in, double ga, double gb, int* iters) {
   double l = 1.0;
   int it = 0;
   bool conv = false;
+  double dp = 0.0;
   for (; it < 200; ++it) {
     double l2 = l * l;
     double p = (l2 + c2) * l2 + c1 * l + c0;
-    double dp = (4.0 * l2 + 2.0 * c2) * l + c1;
+    dp = (4.0 * l2 + 2.0 * c2) * l + c1;
     if (dp == 0.0) break;
     double step = p / dp;
     l -= step;
     if (std::fabs(step) <= 1e-15 * std::fabs(l)) { conv = true; ++it; break; }
   }
+  // Near a double root (near-collinear frames, lambda_1 ~ lambda_2) Newton
+  // only resolves about half the digits: report non-convergence so that the
+  // caller uses the Jacobi eigen-solve instead.
+  if (std::fabs(dp) < 1e-4) conv = false;
   if (iters) *iters = conv ? it : -1;
   return l * g;
 }

@@ -1,0 +1,112 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+
+Independent checks: numpy SVD-Kabsch golden vectors (tests/golden), a
+long-double Jacobi eigen-solve of the 4x4 key matrix, closed forms (rigid
+copies, the 2-atom case, symmetry, weight scaling) and the reference-
+equivalent float port."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from mdsctk_amd import synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = np.load(os.path.join(HERE, "golden", "golden_rmsd.npz"))
+
+
+def test_golden_full_matrix():
+    d = oracle.full_exact(G["rand_ref"], G["rand_fit"], G["rand_w"]) / 10.0
+    assert np.abs(d - G["rand_rmsd_fit"]).max() < 1e-12
+    dn = oracle.full_exact(G["rand_ref"], G["rand_fit"], G["rand_w"], nofit=True) / 10.0
+    assert np.abs(dn - G["rand_rmsd_nofit"]).max() < 1e-12
+    # the fitted value never exceeds the centred-only value
+    assert (d <= dn + 1e-12).all()
+
+
+def test_golden_knn_rows():
+    d, i = oracle.knn_exact(G["rand_ref"], G["rand_fit"], G["rand_w"], 7)
+    assert (i == G["rand_knn7_idx"]).all()
+    assert np.abs(d - G["rand_knn7_dist"]).max() < 1e-11
+
+
+def test_golden_special_geometries():
+    for a, b, w, want in zip(G["geo_a"], G["geo_b"], G["geo_w"], G["geo_rmsd"]):
+        got = oracle.rmsd_nm(a, b, w)
+        assert abs(got - want) < 1e-12, (got, want)
+    # mirror image: a proper rotation cannot superpose it
+    assert G["geo_rmsd"][0] > 0.1
+
+
+def test_qcp_matches_jacobi():
+    rng = np.random.default_rng(3)
+    for t in range(300):
+        a = rng.normal(size=(rng.integers(3, 40), 3))
+        b = a @ np.linalg.qr(rng.normal(size=(3, 3)))[0] + rng.normal(size=a.shape) * rng.choice([1e-6, 1e-3, 0.3, 3])
+        S = a.T @ b
+        ga, gb = (a * a).sum(), (b * b).sum()
+        lam, it = oracle.qcp_lambda(S, ga, gb)
+        lj = oracle.jacobi4_lambda(S)
+        assert it > 0
+        assert abs(lam - lj) <= 1e-12 * max(ga, gb)
+
+
+def test_rigid_copy_is_zero_and_symmetric():
+    rng = np.random.default_rng(4)
+    a = synth.trajectory(1, 50, 4)[0]
+    w = synth.masses(50, "random", 1)
+    R = synth.random_rotations(1, rng)[0]
+    b = (a.astype(np.float64) @ R.T + [1.0, -2.0, 3.0]).astype(np.float32)
+    assert oracle.rmsd_nm(a, b, w) < 1e-6
+    c = synth.trajectory(1, 50, 5, base_seed=4)[0]
+    assert abs(oracle.rmsd_nm(a, c, w) - oracle.rmsd_nm(c, a, w)) < 1e-13
+
+
+def test_two_atom_closed_form():
+    # centred dumbbells of lengths L1, L2, equal weights: RMSD = |L1 - L2| / 2
+    a = np.array([[0, 0, 0], [1.3, 0, 0]], np.float32)
+    b = np.array([[0.5, 0.5, 0.5], [0.5, 0.5 + 0.8, 0.5]], np.float32)
+    assert abs(oracle.rmsd_nm(a, b, np.ones(2, np.float32)) - abs(1.3 - 0.8) / 2) < 1e-7
+
+
+def test_weight_scaling_invariance():
+    x = synth.trajectory(2, 33, 6)
+    w = synth.masses(33, "random", 2)
+    r1 = oracle.rmsd_nm(x[0], x[1], w)
+    r2 = oracle.rmsd_nm(x[0], x[1], w * 8.0)  # exact in float32
+    assert abs(r1 - r2) < 1e-13
+
+
+def test_knn_semantics_clamp_and_rank0():
+    x = synth.trajectory(12, 15, 8)
+    w = synth.masses(15)
+    d, i = oracle.knn_exact(x, x, w, 40)        # k clamps to n_ref - 1
+    assert d.shape == (12, 11)
+    # self query: the dropped rank 0 is the self match, so self never appears
+    assert all(r not in i[r] for r in range(12))
+    d0, i0 = oracle.knn_exact(x, x, w, 3, keep_rank0=True)
+    assert (i0[:, 0] == np.arange(12)).all() and np.abs(d0[:, 0]).max() < 1e-5
+    assert (np.diff(d, axis=1) >= 0).all()
+
+
+def test_port_agrees_with_oracle():
+    """The reference-equivalent float path (cumulative in-place rotation of
+    the fit copy, float rmsdev) stays within the 1e-5 nm tolerance."""
+    ref = synth.trajectory(120, 304, 9)
+    fit = synth.trajectory(30, 304, 10, base_seed=9)
+    w = synth.masses(304, "backbone")
+    k = 10
+    d_o, i_o = oracle.knn_exact(ref, fit, w, k)
+    d_p, i_p = oracle.port_knn_rows(oracle.port_center(ref, w), oracle.port_center(fit, w), w, k)
+    assert np.abs(d_p - d_o).max() < 1e-4          # Angstrom = 1e-5 nm
+    assert (i_p == i_o).mean() > 0.99
+
+
+def test_port_full_rows_layout():
+    ref = synth.trajectory(9, 12, 11)
+    w = synth.masses(12)
+    d, i = oracle.port_knn_rows(oracle.port_center(ref, w), oracle.port_center(ref, w), w, 3, sort=False)
+    assert d.shape == (9, 9) and (i == np.arange(9)[None]).all()
+    full = oracle.full_exact(ref, ref, w)
+    assert np.abs(d - full).max() < 1e-4
