@@ -8,8 +8,9 @@ seeded metastable generator (50 centres, 0.1 nm noise, 5% near-duplicates at
 top-k, FP64 re-score, outputs written to device memory).
 
 Beside the headline number it reports, on the same data, the plain-FP32
-contraction path, a serving-latency probe (1,000-frame queries against a
-100,000 x 500 resident reference, k = 50), the contraction kernel's roofline
+contraction path, the serving form (Poisson arrivals at 200 requests/s of
+1..4096 frames, log-uniform, against a 100,000 x 500 resident reference,
+k = 50, continuous batching; p50/p99 latency), the contraction kernel's roofline
 (HIP events on the engine's stream), accuracy against the FP64 oracle on a
 sample, and the reference-equivalent CPU knn_rms timed on a bounded sample.
 
@@ -100,6 +101,93 @@ def time_fused(ctx, hip, n_fit, k, steps, warmup):
     return wall, stats, dist, idx
 
 
+def run_serving(knn, synth, oracle, prec, args):
+    """Serving form: a resident 100,000 x 500-atom reference set; requests of
+    log-uniform 1..4096 frames arrive as a Poisson process; each is submitted
+    at its arrival time and latency runs from arrival to outputs written."""
+    n_res, na_res, kq, pool_n = 100_000, 500, 50, 16384
+    rng = np.random.default_rng(args.seed + 9)
+    res = synth.trajectory(n_res, na_res, args.seed + 7)
+    w = synth.masses(na_res)
+    pool = synth.trajectory(pool_n, na_res, args.seed + 8, base_seed=args.seed + 7)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serving_rate, int(args.serving_rate * args.serving_seconds * 1.5) + 10))
+    arrivals = arrivals[arrivals < args.serving_seconds]
+    sizes = np.minimum(np.floor(np.exp(rng.uniform(0.0, np.log(4097.0), len(arrivals)))), 4096).astype(np.int64)
+    starts = rng.integers(0, pool_n - sizes + 1)
+    keep = set(rng.choice(len(arrivals), size=min(8, len(arrivals)), replace=False).tolist())
+    log(f"serving: {len(arrivals)} requests over {args.serving_seconds:.0f} s, "
+        f"{sizes.sum()} frames ({sizes.mean():.0f} per request)")
+    lat, done_t, kept = [], [], {}
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(res, w)
+        for nw in (4096, 1, 700, 4096):  # warm-up: sizes the batch slots
+            ctx.wait(ctx.submit(pool[:nw], kq))
+        st0 = ctx.serve_stats()
+        t0 = ctx.clock_ms() + 10.0
+        outstanding = []
+
+        def retire(block=False):
+            still = []
+            for j, r in outstanding:
+                if block:
+                    ctx.wait(r)
+                if r.done or ctx.poll(r):
+                    lat.append(r.info["t_done_ms"] - r.t_arrival)
+                    done_t.append(r.info["t_done_ms"])
+                    if j in keep:
+                        kept[j] = (r.dist, r.idx, r.info["batch_frames"])
+                    r.dist = r.idx = r.xyz = None
+                else:
+                    still.append((j, r))
+            outstanding[:] = still
+
+        for j, (a, nq, s0) in enumerate(zip(arrivals, sizes, starts)):
+            target = t0 + a * 1e3
+            while True:
+                now = ctx.clock_ms()
+                if now >= target:
+                    break
+                retire()
+                if target - now > 1.5:
+                    time.sleep(0.0005)
+            r = ctx.submit(pool[s0:s0 + nq], kq)
+            r.t_arrival = target
+            outstanding.append((j, r))
+        retire(block=True)
+        st1 = ctx.serve_stats()
+        st = {key: st1[key] - st0[key] for key in ("batches", "frames", "busy_ms", "fallback_rows")}
+        st["max_batch_frames"], st["slots"] = st1["max_batch_frames"], st1["slots"]
+        # the kept requests again, each alone: results must be bit-identical
+        same = True
+        for j, (d, i, bf) in kept.items():
+            r = ctx.wait(ctx.submit(pool[starts[j]:starts[j] + sizes[j]], kq))
+            same &= bool((r.dist == d).all() and (r.idx == i).all())
+    # accuracy of the kept requests' first rows vs the FP64 oracle
+    err = 0.0
+    for j, (d, i, _) in kept.items():
+        q = pool[starts[j]:starts[j] + 1]
+        d_o, _ = oracle.knn_exact(res, q, w, kq)
+        err = max(err, float(np.abs(d[:1] - d_o).max()) / 10.0)
+    lat = np.array(lat)
+    span_s = (max(done_t) - t0) / 1e3
+    frames = float(sizes.sum())
+    return {"workload": f"resident {n_res} x {na_res} atoms, Poisson {args.serving_rate:.0f} req/s, "
+                        f"1..4096 frames log-uniform, k={kq}",
+            "duration_s": args.serving_seconds, "requests": int(len(lat)),
+            "frames_per_request": frames / len(lat),
+            "p50_ms": float(np.percentile(lat, 50)), "p90_ms": float(np.percentile(lat, 90)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+            "throughput_pairs_s": frames * n_res / span_s,
+            "offered_pairs_s": frames * n_res / args.serving_seconds,
+            "gpu_busy_frac": st["busy_ms"] / (span_s * 1e3),
+            "batches": st["batches"], "mean_batch_frames": st["frames"] / max(1, st["batches"]),
+            "max_batch_frames": st["max_batch_frames"], "slots": st["slots"],
+            "fallback_rows": st["fallback_rows"],
+            "alone_vs_mix_identical": same, "alone_vs_mix_requests": len(kept),
+            "sample_max_err_nm": err,
+            "mode": "host buffers in and out (PCIe included), latency from scheduled arrival"}
+
+
 def mean_stat(stats, key):
     return float(np.mean([s[key] for s in stats]))
 
@@ -114,8 +202,10 @@ def main():
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16", "fp32"])
     ap.add_argument("--fp32-steps", type=int, default=1, help="steps of the FP32 comparator (0 = skip)")
-    ap.add_argument("--serving-queries", type=int, default=20, help="1k-frame latency probes (0 = skip)")
-    ap.add_argument("--cpu-rows", type=int, default=16, help="fit rows of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--serving-seconds", type=float, default=30.0,
+                    help="length of the Poisson serving run (0 = skip)")
+    ap.add_argument("--serving-rate", type=float, default=200.0, help="mean request arrivals per second")
+    ap.add_argument("--cpu-rows", type=int, default=128, help="fit rows of the CPU baseline sample (0 = skip)")
     ap.add_argument("--check-rows", type=int, default=8, help="rows checked against the FP64 oracle")
     ap.add_argument("--seed", type=int, default=2026)
     args = ap.parse_args()
@@ -126,6 +216,7 @@ def main():
         sys.exit(2)
 
     from mdsctk_amd import knn, synth
+    import oracle
 
     hip = Hip()
     n, natoms, k, prec = args.n_frames, args.natoms, args.k, args.precision
@@ -166,7 +257,6 @@ def main():
     out["fallback_rows"] = int(max(s["n_fallback_rows"] for s in stats))
 
     # accuracy vs the FP64 oracle: 10k output pairs and a few full rows
-    import oracle
     rng = np.random.default_rng(args.seed)
     rows = rng.integers(0, n, 10000)
     cols = rng.integers(0, k, 10000)
@@ -196,25 +286,8 @@ def main():
                             "contract_ms": ms32, "achieved_tflops": a32, "frac_fp32_peak": a32 / 157.3}
         out["speedup_vs_fp32"] = out["value"] / out["fp32_path"]["value"]
 
-    if args.serving_queries > 0:
-        n_res, na_res, q_frames, kq = 100_000, 500, 1000, 50
-        res = synth.trajectory(n_res, na_res, args.seed + 7)
-        qs = synth.trajectory(args.serving_queries * q_frames, na_res, args.seed + 8,
-                              base_seed=args.seed + 7)
-        lat = []
-        with knn.KnnContext(0, prec) as ctx:
-            ctx.set_reference(res, synth.masses(na_res))
-            ctx.query(qs[:q_frames], kq)  # warm-up
-            for j in range(args.serving_queries):
-                t = time.perf_counter()
-                ctx.query(qs[j * q_frames:(j + 1) * q_frames], kq)
-                lat.append((time.perf_counter() - t) * 1e3)
-        lat = np.array(lat)
-        out["serving"] = {"reference": f"{n_res} x {na_res} atoms resident", "query_frames": q_frames,
-                          "k": kq, "requests": len(lat), "p50_ms": float(np.percentile(lat, 50)),
-                          "p99_ms": float(np.percentile(lat, 99)),
-                          "pairs_per_s": q_frames * n_res / (np.mean(lat) * 1e-3),
-                          "mode": "sequential requests, host buffers in and out (PCIe included)"}
+    if args.serving_seconds > 0:
+        out["serving"] = run_serving(knn, synth, oracle, prec, args)
 
     if args.cpu_rows > 0:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

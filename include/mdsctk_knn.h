@@ -23,9 +23,11 @@
  *   - Return value: >= 0 on success, a negative MDSCTK_ERR_* code on
  *     failure; mdsctk_knn_last_error() describes the failure.  The library
  *     never calls exit().
- *   - A context is used from one host thread at a time.  It owns one HIP
- *     stream and all device memory it allocates.  Several contexts may exist
- *     at once (e.g. one per resident reference set).
+ *   - A context is used from one host thread at a time, except for the
+ *     serving calls (submit / poll / wait / serve_stats), which any number
+ *     of host threads may make concurrently.  A context owns its HIP streams
+ *     and all device memory it allocates.  Several contexts may exist at
+ *     once (e.g. one per resident reference set).
  */
 #ifndef MDSCTK_KNN_H
 #define MDSCTK_KNN_H
@@ -64,6 +66,8 @@ extern "C" {
 #define MDSCTK_OPT_HBM_BUDGET 1   /* bytes of device memory the context may hold (0 = no limit) */
 #define MDSCTK_OPT_EPS_SCALE 2    /* multiplier on the candidate window (default 1) */
 #define MDSCTK_OPT_REF_SPLITS 3   /* force the number of reference phases (0 = automatic) */
+#define MDSCTK_OPT_SERVE_SLOTS 4  /* batches in flight at once when serving (1..4, default 2) */
+#define MDSCTK_OPT_SERVE_MAX_FRAMES 5 /* query frames per served batch (default 8192) */
 
 typedef struct mdsctk_knn_ctx mdsctk_knn_ctx;
 
@@ -130,6 +134,72 @@ int mdsctk_knn_query_device(mdsctk_knn_ctx* ctx, const float* d_xyz_nm, int64_t 
  * Limited to n_fit*n_ref <= 6e7. */
 int mdsctk_knn_debug_scores(mdsctk_knn_ctx* ctx, const float* xyz_nm, int64_t n_fit,
                             uint32_t flags, double* approx_s, double* exact_s);
+
+/* ---- Serving: asynchronous requests against the resident reference set.
+ *
+ * The reference tool has no counterpart: this is the out-of-sample
+ * (Nystrom) query form of knn_rms.cpp:231-291 with the reference set kept
+ * resident.  The first submit starts a serving thread that owns
+ * MDSCTK_OPT_SERVE_SLOTS batch slots, each with its own HIP stream.
+ * Pending requests are admitted in arrival order into the next free slot,
+ * up to MDSCTK_OPT_SERVE_MAX_FRAMES frames per batch (clamped so that the
+ * slots fit the HBM budget); a request larger than that is split over
+ * consecutive batches by rows.  A batch is one fused launch sequence over
+ * the concatenated frames of its requests, so requests that arrive while
+ * others run join the next batch and overlap the running one on the device.
+ * Only requests with the same k and flags share a batch.
+ *
+ * Results do not depend on what a request was batched with: every output
+ * value is the FP64 re-score of the certified candidate window, sorted by
+ * (distance, index), and a pair's FP64 score is computed the same way in
+ * every batch.
+ *
+ * Buffers stay owned by the caller and must stay valid until the request
+ * is reported done (xyz_nm may be released once done as well). */
+
+typedef struct mdsctk_knn_request_info {
+  int32_t status;          /* entries per row (>= 0) when done, MDSCTK_ERR_PENDING, or an error */
+  int32_t n_batches;       /* batches the request's rows were spread over */
+  int64_t n_fallback_rows; /* rows that took the exact path */
+  double t_submit_ms;      /* mdsctk_knn_clock_ms() at submit */
+  double t_admit_ms;       /* first rows placed in a batch */
+  double t_done_ms;        /* outputs written */
+  int64_t batch_frames;    /* frames of the largest batch it ran in */
+} mdsctk_knn_request_info;
+
+typedef struct mdsctk_knn_serve_stats {
+  int64_t requests_done;
+  int64_t batches;
+  int64_t frames;          /* query frames served */
+  int64_t pairs;           /* frames * n_ref */
+  int64_t max_batch_frames;
+  int64_t fallback_rows;
+  int32_t slots;
+  int32_t slot_frames;     /* frames per batch after the HBM budget clamp */
+  double busy_ms;          /* wall time with at least one batch in flight */
+} mdsctk_knn_serve_stats;
+
+/* Queue a query (same arguments and output layout as mdsctk_knn_query;
+ * NULL xyz_nm and MDSCTK_KNN_FULL_ROWS are not accepted).  Returns 0 and
+ * the request id, or a negative error. */
+int mdsctk_knn_submit(mdsctk_knn_ctx* ctx, const float* xyz_nm, int64_t n_fit, int32_t k,
+                      uint32_t flags, double* out_dist, int32_t* out_idx, int64_t* request_id);
+
+/* State of a request: entries per row when done, MDSCTK_ERR_PENDING while
+ * queued or running, or the request's error code.  A request reported done
+ * or failed is retired (its id is then unknown: MDSCTK_ERR_STATE).  `info`
+ * may be NULL. */
+int mdsctk_knn_poll(mdsctk_knn_ctx* ctx, int64_t request_id, mdsctk_knn_request_info* info);
+
+/* Block until the request is done or failed; then as mdsctk_knn_poll. */
+int mdsctk_knn_wait(mdsctk_knn_ctx* ctx, int64_t request_id, mdsctk_knn_request_info* info);
+
+/* Counters of the serving loop since the first submit. */
+int mdsctk_knn_serve_stats_get(mdsctk_knn_ctx* ctx, mdsctk_knn_serve_stats* out);
+
+/* The host clock the request times use: milliseconds since the context was
+ * created (steady clock). */
+double mdsctk_knn_clock_ms(const mdsctk_knn_ctx* ctx);
 
 /* Statistics of the last completed query. */
 int mdsctk_knn_get_stats(const mdsctk_knn_ctx* ctx, mdsctk_knn_stats* out);

@@ -32,7 +32,7 @@ DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
           f"-I{os.path.join(ROOT, 'include')}"]
 
-LIB_SOURCES = ["knn_kernels.hip", "api.cpp", "xtc.cpp", "topology.cpp"]
+LIB_SOURCES = ["knn_kernels.hip", "api.cpp", "serve.cpp", "xtc.cpp", "topology.cpp"]
 
 
 def _run(cmd, verbose):

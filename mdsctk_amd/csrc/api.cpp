@@ -1,22 +1,20 @@
 // C ABI of the knn_rms engine (include/mdsctk_knn.h): context, device
 // memory accounting, and the per-query orchestration of the gfx950 kernels.
-#include <hip/hip_runtime.h>
-
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
-#include "../../include/mdsctk_knn.h"
-#include "knn_device.h"
+#include "engine.h"
 
-using namespace mdsctk;
+namespace mdsctk {
 
 namespace {
+
+thread_local std::string* tl_err_sink = nullptr;
 
 // Candidate window: |s~ - s| <= kappa (G_q + max G_ref) for the approximate
 // score s~ of each contraction form (s = G_q + G_r - 2 lambda).  kappa is an
@@ -36,44 +34,24 @@ double kappa_for(int prec, int natoms) {
   }
 }
 
+size_t operand_bytes(int prec, int n_tiles, int n_groups) {
+  return ((size_t)n_tiles * n_groups + kOperandTailGroups) * blocks_per_group(prec) * 64 * 16;
+}
+
+int padded_tiles(int64_t n) {
+  const int64_t tiles = (n + kTile - 1) / kTile;
+  return (int)((tiles + kTileAlign - 1) / kTileAlign * kTileAlign);
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
 }  // namespace
 
-struct mdsctk_knn_ctx {
-  int device = 0;
-  int prec = kFP32;
-  int n_cu = 256;
-  hipStream_t st = nullptr;
-  std::string err;
-  size_t budget = 0;  // 0 = unlimited
-  size_t held = 0;
-  std::unordered_map<void*, size_t> allocs;
-  double eps_scale = 1.0;
-  int forced_phases = 0;
-
-  DevWeights w;
-  DevFrames ref;
-  bool has_ref = false;
-  DevFrames fit;  // query frames (grown on demand)
-  size_t fit_cap_frames = 0;
-  int fit_natoms = 0;
-
-  // query workspace (grown on demand)
-  void* ws_lists = nullptr;
-  size_t ws_lists_bytes = 0;
-  void* ws_small = nullptr;  // per-query arrays
-  size_t ws_small_bytes = 0;
-  double* gmax = nullptr;
-  int* n_cand = nullptr;
-  void* ws_out = nullptr;  // device outputs for the host API
-  size_t ws_out_bytes = 0;
-  void* ws_rows = nullptr;  // exact-row scratch
-  size_t ws_rows_bytes = 0;
-
-  hipEvent_t ev[6] = {};
-  mdsctk_knn_stats stats = {};
-};
-
-namespace {
+void set_error_sink(std::string* sink) { tl_err_sink = sink; }
 
 int fail(mdsctk_knn_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -81,7 +59,10 @@ int fail(mdsctk_knn_ctx* c, int code, const char* fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  if (c) c->err = buf;
+  if (tl_err_sink)
+    *tl_err_sink = buf;
+  else if (c)
+    c->err = buf;
   return code;
 }
 
@@ -100,27 +81,35 @@ int dalloc(mdsctk_knn_ctx* c, void** p, size_t bytes, const char* what) {
   *p = nullptr;
   if (bytes == 0) bytes = 256;
   bytes = (bytes + 255) & ~size_t(255);
-  if (c->budget && c->held + bytes > c->budget)
-    return fail(c, MDSCTK_ERR_NOMEM,
-                "%s: %zu bytes would exceed the HBM budget (%zu held of %zu)", what, bytes,
-                c->held, c->budget);
+  {
+    std::lock_guard<std::mutex> g(c->mem_mu);
+    if (c->budget && c->held + bytes > c->budget)
+      return fail(c, MDSCTK_ERR_NOMEM,
+                  "%s: %zu bytes would exceed the HBM budget (%zu held of %zu)", what, bytes,
+                  c->held, c->budget);
+    c->held += bytes;  // reserved before the allocation so that threads cannot overshoot
+  }
   hipError_t e = hipMalloc(p, bytes);
+  std::lock_guard<std::mutex> g(c->mem_mu);
   if (e != hipSuccess) {
     *p = nullptr;
+    c->held -= bytes;
     return fail(c, MDSCTK_ERR_NOMEM, "%s: hipMalloc(%zu) failed: %s", what, bytes,
                 hipGetErrorString(e));
   }
-  c->held += bytes;
   c->allocs[*p] = bytes;
   return 0;
 }
 
 void dfree(mdsctk_knn_ctx* c, void* p) {
   if (!p) return;
-  auto it = c->allocs.find(p);
-  if (it != c->allocs.end()) {
-    c->held -= it->second;
-    c->allocs.erase(it);
+  {
+    std::lock_guard<std::mutex> g(c->mem_mu);
+    auto it = c->allocs.find(p);
+    if (it != c->allocs.end()) {
+      c->held -= it->second;
+      c->allocs.erase(it);
+    }
   }
   (void)hipFree(p);
 }
@@ -137,6 +126,8 @@ int ensure(mdsctk_knn_ctx* c, void** p, size_t* have, size_t need, const char* w
   return 0;
 }
 
+namespace {
+
 void free_frames(mdsctk_knn_ctx* c, DevFrames& f) {
   if (f.owns_xyz) dfree(c, f.xyz);
   dfree(c, f.cen);
@@ -146,17 +137,13 @@ void free_frames(mdsctk_knn_ctx* c, DevFrames& f) {
   f = DevFrames();
 }
 
-size_t operand_bytes(int prec, int n_tiles, int n_groups) {
-  return ((size_t)n_tiles * n_groups + kOperandTailGroups) * blocks_per_group(prec) * 64 * 16;
-}
-
 // Allocate the derived arrays of a frame set (xyz handled by the caller).
 int alloc_frames(mdsctk_knn_ctx* c, DevFrames& f, int64_t n, int natoms, int prec) {
   f.n = n;
   f.natoms = natoms;
   f.prec = prec == kExact ? kFP32 : prec;
-  const int64_t tiles = (n + kTile - 1) / kTile;
-  f.n_tiles = (int)((tiles + kTileAlign - 1) / kTileAlign * kTileAlign);
+  f.n_tiles = padded_tiles(n);
+  f.cap_tiles = f.n_tiles;
   const int ga = group_atoms(f.prec);
   f.n_groups = (natoms + ga - 1) / ga;
   const size_t rows = (size_t)f.n_tiles * kTile;
@@ -170,23 +157,22 @@ int alloc_frames(mdsctk_knn_ctx* c, DevFrames& f, int64_t n, int natoms, int pre
   return 0;
 }
 
-int prep_frames(mdsctk_knn_ctx* c, DevFrames& f) {
+int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st) {
   if (c->prec == kExact) {
-    // The exact form needs only centroids and G; run the FP32 prep without
-    // an operand buffer is not possible, so compute them through a
-    // temporary operand.
+    // The exact form needs only centroids and G; the prep pass writes an
+    // operand too, so give it a temporary one.
     void* tmp = nullptr;
     int rc = dalloc(c, &tmp, operand_bytes(kFP32, f.n_tiles, f.n_groups), "prep scratch");
     if (rc) return rc;
     f.op = (uint4*)tmp;
-    hipError_t e = launch_prep(f, c->w, c->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    hipError_t e = launch_prep(f, c->w, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     f.op = nullptr;
     dfree(c, tmp);
     if (e != hipSuccess) return hip_fail(c, e, "prep");
     return 0;
   }
-  HIP_TRY(c, launch_prep(f, c->w, c->st));
+  HIP_TRY(c, launch_prep(f, c->w, st));
   return 0;
 }
 
@@ -208,9 +194,10 @@ int set_weights(mdsctk_knn_ctx* c, const float* weights, int natoms) {
   int rc;
   if ((rc = dalloc(c, (void**)&c->w.w, natoms * sizeof(float), "weights"))) return rc;
   if ((rc = dalloc(c, (void**)&c->w.sw, natoms * sizeof(float), "weights"))) return rc;
-  HIP_TRY(c, hipMemcpyAsync(c->w.w, w.data(), natoms * sizeof(float), hipMemcpyHostToDevice, c->st));
-  HIP_TRY(c, hipMemcpyAsync(c->w.sw, sw.data(), natoms * sizeof(float), hipMemcpyHostToDevice, c->st));
-  HIP_TRY(c, hipStreamSynchronize(c->st));
+  hipStream_t st = c->main.st;
+  HIP_TRY(c, hipMemcpyAsync(c->w.w, w.data(), natoms * sizeof(float), hipMemcpyHostToDevice, st));
+  HIP_TRY(c, hipMemcpyAsync(c->w.sw, sw.data(), natoms * sizeof(float), hipMemcpyHostToDevice, st));
+  HIP_TRY(c, hipStreamSynchronize(st));
   c->w.natoms = natoms;
   c->w.W = W;
   return 0;
@@ -223,6 +210,9 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
     return fail(c, MDSCTK_ERR_ARG, "set_reference: need frames (n_ref=%lld natoms=%d)",
                 (long long)n_ref, natoms);
   if (n_ref > INT32_MAX) return fail(c, MDSCTK_ERR_ARG, "n_ref exceeds the int32 index range");
+  if (!server_idle(c))
+    return fail(c, MDSCTK_ERR_STATE, "set_reference: served requests are still outstanding");
+  server_stop(c);
   HIP_TRY(c, hipSetDevice(c->device));
   free_frames(c, c->ref);
   c->has_ref = false;
@@ -231,77 +221,184 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
   const size_t xbytes = (size_t)n_ref * natoms * 3 * sizeof(float);
   if ((rc = dalloc(c, (void**)&c->ref.xyz, xbytes, "reference coordinates"))) return rc;
   HIP_TRY(c, hipMemcpyAsync(c->ref.xyz, xyz, xbytes,
-                            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
+                            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                            c->main.st));
   c->ref.owns_xyz = true;
   if ((rc = alloc_frames(c, c->ref, n_ref, natoms, c->prec))) return rc;
-  if ((rc = prep_frames(c, c->ref))) return rc;
-  HIP_TRY(c, hipStreamSynchronize(c->st));
+  if ((rc = prep_frames(c, c->ref, c->main.st))) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->main.st));
   c->has_ref = true;
   return 0;
 }
 
-// Upload (or alias) the query frames and run their prep pass.
-int stage_queries(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int64_t n_fit,
-                  DevFrames** out) {
-  if (!xyz) {
-    *out = &c->ref;
-    return 0;
-  }
-  DevFrames& f = c->fit;
-  const int natoms = c->ref.natoms;
-  if (f.n_tiles == 0 || (size_t)n_fit > c->fit_cap_frames || c->fit_natoms != natoms ||
-      f.prec != (c->prec == kExact ? kFP32 : c->prec)) {
-    free_frames(c, f);
-    int rc;
-    if ((rc = dalloc(c, (void**)&f.xyz, (size_t)n_fit * natoms * 3 * sizeof(float),
-                     "query coordinates")))
-      return rc;
-    f.owns_xyz = true;
-    if ((rc = alloc_frames(c, f, n_fit, natoms, c->prec))) return rc;
-    c->fit_cap_frames = (size_t)n_fit;
-    c->fit_natoms = natoms;
-  }
-  // Reuse the allocation: a smaller batch only changes the row count.
-  f.n = n_fit;
-  const size_t xbytes = (size_t)n_fit * natoms * 3 * sizeof(float);
-  HIP_TRY(c, hipMemcpyAsync(f.xyz, xyz, xbytes,
-                            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
-  int rc = prep_frames(c, f);
-  if (rc) return rc;
-  *out = &f;
+}  // namespace
+
+int slot_init(mdsctk_knn_ctx* c, Slot& s) {
+  HIP_TRY(c, hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+  for (auto& e : s.ev) HIP_TRY(c, hipEventCreate(&e));
+  int rc;
+  if ((rc = dalloc(c, (void**)&s.gmax, 256, "scratch"))) return rc;
+  if ((rc = dalloc(c, (void**)&s.n_cand, 256, "scratch"))) return rc;
   return 0;
 }
 
-float elapsed(hipEvent_t a, hipEvent_t b) {
-  float ms = 0.f;
-  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
-  return ms;
+void slot_release(mdsctk_knn_ctx* c, Slot& s) {
+  if (s.st) (void)hipStreamSynchronize(s.st);
+  free_frames(c, s.fit);
+  dfree(c, s.ws_lists);
+  dfree(c, s.ws_small);
+  dfree(c, s.gmax);
+  dfree(c, s.n_cand);
+  dfree(c, s.ws_out);
+  dfree(c, s.ws_rows);
+  for (auto& e : s.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s.st) (void)hipStreamDestroy(s.st);
+  s = Slot();
+}
+
+double clock_ms(const mdsctk_knn_ctx* c) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c->t0)
+      .count();
+}
+
+int resolve_shape(mdsctk_knn_ctx* c, int32_t k, uint32_t flags, QueryShape* q) {
+  const int64_t n_ref = c->ref.n;
+  q->nofit = flags & MDSCTK_KNN_NOFIT;
+  q->full = flags & MDSCTK_KNN_FULL_ROWS;
+  q->rank_off = (flags & MDSCTK_KNN_KEEP_RANK0) ? 0 : 1;
+  if (!q->full && k < 0) return fail(c, MDSCTK_ERR_ARG, "query: k must be >= 0");
+  // knn_rms.cpp:224-225 clamps k to n_ref-1; k+1 follows the clamp here.
+  q->k_out = q->full ? (int)n_ref : (int)std::min<int64_t>(k, n_ref - 1);
+  if (!q->full && q->rank_off == 0) q->k_out = (int)std::min<int64_t>(k, n_ref);
+  q->k1 = q->full ? 0 : q->k_out + q->rank_off;
+  if (!q->full && q->k1 > 2048)
+    return fail(c, MDSCTK_ERR_ARG, "query: k=%d exceeds the supported maximum 2047", k);
+  q->fused = !q->full && c->prec != kExact && list_capacity_for(q->k1) > 0;
+  return 0;
+}
+
+int slot_frames(mdsctk_knn_ctx* c, Slot& s, int64_t n_fit) {
+  DevFrames& f = s.fit;
+  const int natoms = c->ref.natoms;
+  const int want_prec = c->prec == kExact ? kFP32 : c->prec;
+  if (f.cap_tiles == 0 || (size_t)n_fit > s.fit_cap_frames || s.fit_natoms != natoms ||
+      f.prec != want_prec) {
+    free_frames(c, f);
+    s.fit_cap_frames = 0;
+    int rc;
+    if ((rc = dalloc(c, (void**)&f.xyz, (size_t)std::max<int64_t>(n_fit, 1) * natoms * 3 * sizeof(float),
+                     "query coordinates")))
+      return rc;
+    f.owns_xyz = true;
+    if ((rc = alloc_frames(c, f, std::max<int64_t>(n_fit, 1), natoms, c->prec))) return rc;
+    s.fit_cap_frames = (size_t)std::max<int64_t>(n_fit, 1);
+    s.fit_natoms = natoms;
+  }
+  // Reuse the allocation: a smaller batch only changes the row and tile
+  // counts (the kernels walk n_tiles; the slack beyond it is never read).
+  f.n = n_fit;
+  f.n_tiles = std::max(kTileAlign, padded_tiles(n_fit));
+  return 0;
+}
+
+int prep_slot(mdsctk_knn_ctx* c, Slot& s) { return prep_frames(c, s.fit, s.st); }
+
+int contract_grid(const mdsctk_knn_ctx* c) {
+  // workgroups resident at once: two 4-wave or one 8-wave per CU
+  const int ft = contract_fit_tiles(c->prec == kExact ? kFP32 : c->prec);
+  return std::max(8, (c->n_cu * (ft == 2 ? 2 : 1)) & ~7);
+}
+
+int fused_phases(const mdsctk_knn_ctx* c, int fit_tiles) {
+  const int ft = contract_fit_tiles(c->prec == kExact ? kFP32 : c->prec);
+  const int fit_blocks = std::max(1, fit_tiles / ft);
+  const int ref_pairs = c->ref.n_tiles / kRefTilesPerWG;
+  const int grid = contract_grid(c);
+  int phases = c->forced_phases;
+  if (phases <= 0) {
+    // enough (fit block, phase) items to fill the grid, at least 8 phases
+    phases = std::max(8, (grid + fit_blocks - 1) / fit_blocks);
+    int p2 = 1;
+    while (p2 < phases) p2 <<= 1;
+    phases = p2;
+  }
+  return std::max(1, std::min(phases, ref_pairs));
+}
+
+size_t fused_list_entries(const mdsctk_knn_ctx* c, int64_t n_fit, int k1) {
+  const int phases = fused_phases(c, std::max(kTileAlign, padded_tiles(n_fit)));
+  return (size_t)n_fit * phases * 4 * list_capacity_for(k1);
+}
+
+int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
+                 FusedInfo* info) {
+  DevFrames& fit = s.fit;
+  const int64_t n_fit = fit.n;
+  const int cap = list_capacity_for(q.k1);
+  const int grid = contract_grid(c);
+  const int phases = fused_phases(c, fit.n_tiles);
+  Lists L;
+  L.nl = phases * 4;
+  L.cap = cap;
+  const size_t n_hdr = (size_t)n_fit * L.nl;
+  const size_t ent_bytes = n_hdr * cap * sizeof(uint2);
+  int rc = ensure(c, &s.ws_lists, &s.ws_lists_bytes, ent_bytes, "candidate lists");
+  if (rc) return rc;
+  L.ent = (uint2*)s.ws_lists;
+  const size_t small = n_hdr * (sizeof(int) + sizeof(float)) +
+                       (size_t)n_fit * (sizeof(unsigned) + sizeof(float) + sizeof(int)) + 1024;
+  rc = ensure(c, &s.ws_small, &s.ws_small_bytes, small, "list headers");
+  if (rc) return rc;
+  char* sp = (char*)s.ws_small;
+  L.cnt = (int*)sp; sp += n_hdr * sizeof(int);
+  L.ovf = (float*)sp; sp += n_hdr * sizeof(float);
+  L.thr = (unsigned*)sp; sp += n_fit * sizeof(unsigned);
+  float* eps2 = (float*)sp; sp += n_fit * sizeof(float);
+  int* d_fb = (int*)sp;
+  const double kappa = kappa_for(c->prec, c->ref.natoms) * c->eps_scale;
+  HIP_TRY(c, launch_init_lists(L, n_fit, s.st));
+  HIP_TRY(c, launch_eps(fit, c->ref, kappa, eps2, s.gmax, s.st));
+  HIP_TRY(c, hipMemsetAsync(s.n_cand, 0, sizeof(int), s.st));
+  HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
+  HIP_TRY(c, launch_contract(c->ref, fit, L, eps2, q.k1, phases, q.nofit, nullptr, grid, s.st));
+  HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
+  HIP_TRY(c, launch_merge(c->ref, fit, c->w, L, eps2, q.k1, q.k_out, q.rank_off, q.nofit, d_dist,
+                          d_idx, d_fb, s.n_cand, s.st));
+  HIP_TRY(c, hipEventRecord(s.ev[4], s.st));
+  info->phases = phases;
+  info->cap = cap;
+  info->kappa = kappa;
+  info->d_fallback = d_fb;
+  return 0;
 }
 
 // Exact FP64 path for the listed fit rows: scores of whole rows in batches,
 // then (s, index) selection.  rows == nullptr means rows [0, nrows).
-int exact_rows_select(mdsctk_knn_ctx* c, DevFrames& fit, const int* h_rows, int nrows, int k1,
-                      int k_out, int rank_off, bool nofit, double* d_dist, int* d_idx) {
+int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const int* h_rows,
+                      int nrows, const QueryShape& q, double* d_dist, int* d_idx) {
   const int64_t n_ref = c->ref.n;
   const size_t row_bytes = (size_t)n_ref * sizeof(double);
   int batch = (int)std::max<int64_t>(1, std::min<int64_t>(nrows, (int64_t)(1ull << 30) / row_bytes));
-  int rc = ensure(c, &c->ws_rows, &c->ws_rows_bytes, row_bytes * batch + sizeof(int) * batch,
+  int rc = ensure(c, &s.ws_rows, &s.ws_rows_bytes, row_bytes * batch + sizeof(int) * batch,
                   "exact-row scratch");
   if (rc) return rc;
-  double* srow = (double*)c->ws_rows;
-  int* drows = (int*)((char*)c->ws_rows + row_bytes * batch);
+  double* srow = (double*)s.ws_rows;
+  int* drows = (int*)((char*)s.ws_rows + row_bytes * batch);
   std::vector<int> ids(batch);
   for (int b0 = 0; b0 < nrows; b0 += batch) {
     const int nb = std::min(batch, nrows - b0);
     for (int i = 0; i < nb; ++i) ids[i] = h_rows ? h_rows[b0 + i] : b0 + i;
-    HIP_TRY(c, hipMemcpyAsync(drows, ids.data(), nb * sizeof(int), hipMemcpyHostToDevice, c->st));
-    HIP_TRY(c, launch_exact_rows(c->ref, fit, c->w, drows, nb, nofit, srow, nullptr, 0, c->st));
-    HIP_TRY(c, launch_select_rows(srow, n_ref, drows, nb, c->w.W, k1, k_out, rank_off, d_dist,
-                                  d_idx, c->st));
-    HIP_TRY(c, hipStreamSynchronize(c->st));
+    HIP_TRY(c, hipMemcpyAsync(drows, ids.data(), nb * sizeof(int), hipMemcpyHostToDevice, s.st));
+    HIP_TRY(c, launch_exact_rows(c->ref, fit, c->w, drows, nb, q.nofit, srow, nullptr, 0, s.st));
+    HIP_TRY(c, launch_select_rows(srow, n_ref, drows, nb, c->w.W, q.k1, q.k_out, q.rank_off,
+                                  d_dist, d_idx, s.st));
+    HIP_TRY(c, hipStreamSynchronize(s.st));
   }
   return 0;
 }
+
+namespace {
 
 int query_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int64_t n_fit, int32_t k,
                uint32_t flags, double* out_dist, int32_t* out_idx, bool out_on_device) {
@@ -310,146 +407,113 @@ int query_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int64_t n_fi
   if (xyz == nullptr) n_fit = c->ref.n;
   if (n_fit < 0) return fail(c, MDSCTK_ERR_ARG, "query: negative frame count");
   if (n_fit > INT32_MAX) return fail(c, MDSCTK_ERR_ARG, "query: too many frames");
-  const bool nofit = flags & MDSCTK_KNN_NOFIT;
-  const bool full = flags & MDSCTK_KNN_FULL_ROWS;
-  const int rank_off = (flags & MDSCTK_KNN_KEEP_RANK0) ? 0 : 1;
-  const int64_t n_ref = c->ref.n;
-  if (!full && k < 0) return fail(c, MDSCTK_ERR_ARG, "query: k must be >= 0");
-  // knn_rms.cpp:224-225 clamps k to n_ref-1; k+1 follows the clamp here.
-  int k_out = full ? (int)n_ref : (int)std::min<int64_t>(k, n_ref - 1);
-  if (!full && rank_off == 0) k_out = (int)std::min<int64_t>(k, n_ref);
-  const int k1 = full ? 0 : k_out + rank_off;
-  if (n_fit == 0 || k_out == 0) {
-    c->stats = mdsctk_knn_stats();
-    return k_out;
-  }
-  if (!full && k1 > 2048)
-    return fail(c, MDSCTK_ERR_ARG, "query: k=%d exceeds the supported maximum 2047", k);
-  HIP_TRY(c, hipSetDevice(c->device));
-  mdsctk_knn_stats S = {};
-  S.n_pairs = n_fit * n_ref;
-  HIP_TRY(c, hipEventRecord(c->ev[0], c->st));
-
-  DevFrames* fit = nullptr;
-  int rc = stage_queries(c, xyz, on_device, n_fit, &fit);
+  QueryShape q;
+  int rc = resolve_shape(c, k, flags, &q);
   if (rc) return rc;
-  HIP_TRY(c, hipEventRecord(c->ev[1], c->st));
+  if (n_fit == 0 || q.k_out == 0) {
+    c->stats = mdsctk_knn_stats();
+    return q.k_out;
+  }
+  HIP_TRY(c, hipSetDevice(c->device));
+  Slot& s = c->main;
+  mdsctk_knn_stats S = {};
+  S.n_pairs = n_fit * c->ref.n;
+  HIP_TRY(c, hipEventRecord(s.ev[0], s.st));
 
-  const size_t out_elems = (size_t)n_fit * k_out;
+  // Stage (or alias) the query frames and run their prep pass.
+  const DevFrames* fitp = &c->ref;
+  if (xyz) {
+    if ((rc = slot_frames(c, s, n_fit))) return rc;
+    const size_t xbytes = (size_t)n_fit * c->ref.natoms * 3 * sizeof(float);
+    HIP_TRY(c, hipMemcpyAsync(s.fit.xyz, xyz, xbytes,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s.st));
+    if ((rc = prep_slot(c, s))) return rc;
+    fitp = &s.fit;
+  }
+  HIP_TRY(c, hipEventRecord(s.ev[1], s.st));
+
+  const size_t out_elems = (size_t)n_fit * q.k_out;
   double* d_dist = out_dist;
   int* d_idx = out_idx;
   if (!out_on_device) {
-    rc = ensure(c, &c->ws_out, &c->ws_out_bytes, out_elems * (sizeof(double) + sizeof(int)),
+    rc = ensure(c, &s.ws_out, &s.ws_out_bytes, out_elems * (sizeof(double) + sizeof(int)),
                 "output buffers");
     if (rc) return rc;
-    d_dist = (double*)c->ws_out;
-    d_idx = (int*)((char*)c->ws_out + out_elems * sizeof(double));
+    d_dist = (double*)s.ws_out;
+    d_idx = (int*)((char*)s.ws_out + out_elems * sizeof(double));
   }
 
   float ms_contract = 0, ms_merge = 0, ms_fallback = 0;
-  if (full) {
-    HIP_TRY(c, launch_exact_rows(c->ref, *fit, c->w, nullptr, (int)n_fit, nofit, nullptr, d_dist,
-                                 n_ref, c->st));
-    HIP_TRY(c, launch_identity_rows(n_fit, n_ref, d_idx, c->st));
-    HIP_TRY(c, hipEventRecord(c->ev[2], c->st));
-    HIP_TRY(c, hipEventRecord(c->ev[3], c->st));
-    HIP_TRY(c, hipEventSynchronize(c->ev[3]));
-    ms_contract = elapsed(c->ev[1], c->ev[2]);
-  } else if (c->prec == kExact || list_capacity_for(k1) == 0) {
-    HIP_TRY(c, hipEventRecord(c->ev[2], c->st));
-    rc = exact_rows_select(c, *fit, nullptr, (int)n_fit, k1, k_out, rank_off, nofit, d_dist, d_idx);
+  if (q.full) {
+    HIP_TRY(c, launch_exact_rows(c->ref, *fitp, c->w, nullptr, (int)n_fit, q.nofit, nullptr,
+                                 d_dist, c->ref.n, s.st));
+    HIP_TRY(c, launch_identity_rows(n_fit, c->ref.n, d_idx, s.st));
+    HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
+    HIP_TRY(c, hipEventSynchronize(s.ev[2]));
+    ms_contract = elapsed(s.ev[1], s.ev[2]);
+  } else if (!q.fused) {
+    HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
+    rc = exact_rows_select(c, s, *fitp, nullptr, (int)n_fit, q, d_dist, d_idx);
     if (rc) return rc;
-    HIP_TRY(c, hipEventRecord(c->ev[3], c->st));
-    HIP_TRY(c, hipEventSynchronize(c->ev[3]));
-    ms_fallback = elapsed(c->ev[2], c->ev[3]);
+    HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
+    HIP_TRY(c, hipEventSynchronize(s.ev[3]));
+    ms_fallback = elapsed(s.ev[2], s.ev[3]);
     S.n_fallback_rows = n_fit;
   } else {
-    // Fused path.
-    const int cap = list_capacity_for(k1);
-    const int ft = contract_fit_tiles(fit->prec);
-    const int fit_blocks = fit->n_tiles / ft;
-    const int ref_pairs = c->ref.n_tiles / kRefTilesPerWG;
-    // workgroups resident at once: two 4-wave or one 8-wave per CU
-    const int grid = std::max(8, (c->n_cu * (ft == 2 ? 2 : 1)) & ~7);
-    int phases = c->forced_phases;
-    if (phases <= 0) {
-      phases = std::max(8, (grid + fit_blocks - 1) / fit_blocks);
-      int p2 = 1;
-      while (p2 < phases) p2 <<= 1;
-      phases = p2;
-    }
-    phases = std::max(1, std::min(phases, ref_pairs));
-    Lists L;
-    L.nl = phases * 4;
-    L.cap = cap;
-    const size_t n_hdr = (size_t)n_fit * L.nl;
-    const size_t ent_bytes = n_hdr * cap * sizeof(uint2);
-    rc = ensure(c, &c->ws_lists, &c->ws_lists_bytes, ent_bytes, "candidate lists");
+    // The fused pipeline runs on the slot's frames; a self query points the
+    // slot at the resident set for its duration.
+    DevFrames saved = s.fit;
+    if (!xyz) s.fit = c->ref;
+    FusedInfo info;
+    rc = launch_fused(c, s, q, d_dist, d_idx, &info);
+    if (!xyz) s.fit = saved;
     if (rc) return rc;
-    L.ent = (uint2*)c->ws_lists;
-    const size_t small = n_hdr * (sizeof(int) + sizeof(float)) +
-                         (size_t)n_fit * (sizeof(unsigned) + sizeof(float) + sizeof(int)) + 1024;
-    rc = ensure(c, &c->ws_small, &c->ws_small_bytes, small, "list headers");
-    if (rc) return rc;
-    char* sp = (char*)c->ws_small;
-    L.cnt = (int*)sp; sp += n_hdr * sizeof(int);
-    L.ovf = (float*)sp; sp += n_hdr * sizeof(float);
-    L.thr = (unsigned*)sp; sp += n_fit * sizeof(unsigned);
-    float* eps2 = (float*)sp; sp += n_fit * sizeof(float);
-    int* d_fb = (int*)sp;
-    const double kappa = kappa_for(c->prec, c->ref.natoms) * c->eps_scale;
-    HIP_TRY(c, launch_init_lists(L, n_fit, c->st));
-    HIP_TRY(c, launch_eps(*fit, c->ref, kappa, eps2, c->gmax, c->st));
-    HIP_TRY(c, hipMemsetAsync(c->n_cand, 0, sizeof(int), c->st));
-    HIP_TRY(c, hipEventRecord(c->ev[2], c->st));
-    HIP_TRY(c, launch_contract(c->ref, *fit, L, eps2, k1, phases, nofit, nullptr, grid, c->st));
-    HIP_TRY(c, hipEventRecord(c->ev[3], c->st));
-    HIP_TRY(c, launch_merge(c->ref, *fit, c->w, L, eps2, k1, k_out, rank_off, nofit, d_dist,
-                            d_idx, d_fb, c->n_cand, c->st));
-    HIP_TRY(c, hipEventRecord(c->ev[4], c->st));
     std::vector<int> fb(n_fit);
     int ncand = 0;
-    HIP_TRY(c, hipMemcpyAsync(fb.data(), d_fb, n_fit * sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIP_TRY(c, hipMemcpyAsync(&ncand, c->n_cand, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIP_TRY(c, hipStreamSynchronize(c->st));
-    ms_contract = elapsed(c->ev[2], c->ev[3]);
-    ms_merge = elapsed(c->ev[3], c->ev[4]);
+    HIP_TRY(c, hipMemcpyAsync(fb.data(), info.d_fallback, n_fit * sizeof(int),
+                              hipMemcpyDeviceToHost, s.st));
+    HIP_TRY(c, hipMemcpyAsync(&ncand, s.n_cand, sizeof(int), hipMemcpyDeviceToHost, s.st));
+    HIP_TRY(c, hipStreamSynchronize(s.st));
+    ms_contract = elapsed(s.ev[2], s.ev[3]);
+    ms_merge = elapsed(s.ev[3], s.ev[4]);
     S.n_candidates = ncand;
-    S.ref_splits = phases;
-    S.list_capacity = cap;
-    S.eps_rel = kappa;
+    S.ref_splits = info.phases;
+    S.list_capacity = info.cap;
+    S.eps_rel = info.kappa;
     std::vector<int> rows;
     for (int64_t i = 0; i < n_fit; ++i)
       if (fb[i]) rows.push_back((int)i);
     if (!rows.empty()) {
-      HIP_TRY(c, hipEventRecord(c->ev[2], c->st));
-      rc = exact_rows_select(c, *fit, rows.data(), (int)rows.size(), k1, k_out, rank_off, nofit,
-                             d_dist, d_idx);
+      HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
+      rc = exact_rows_select(c, s, *fitp, rows.data(), (int)rows.size(), q, d_dist, d_idx);
       if (rc) return rc;
-      HIP_TRY(c, hipEventRecord(c->ev[3], c->st));
-      HIP_TRY(c, hipEventSynchronize(c->ev[3]));
-      ms_fallback = elapsed(c->ev[2], c->ev[3]);
+      HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
+      HIP_TRY(c, hipEventSynchronize(s.ev[3]));
+      ms_fallback = elapsed(s.ev[2], s.ev[3]);
       S.n_fallback_rows = (int64_t)rows.size();
     }
   }
   if (!out_on_device) {
     HIP_TRY(c, hipMemcpyAsync(out_dist, d_dist, out_elems * sizeof(double), hipMemcpyDeviceToHost,
-                              c->st));
+                              s.st));
     HIP_TRY(c, hipMemcpyAsync(out_idx, d_idx, out_elems * sizeof(int), hipMemcpyDeviceToHost,
-                              c->st));
+                              s.st));
   }
-  HIP_TRY(c, hipEventRecord(c->ev[5], c->st));
-  HIP_TRY(c, hipEventSynchronize(c->ev[5]));
-  S.ms_prep = elapsed(c->ev[0], c->ev[1]);
+  HIP_TRY(c, hipEventRecord(s.ev[5], s.st));
+  HIP_TRY(c, hipEventSynchronize(s.ev[5]));
+  S.ms_prep = elapsed(s.ev[0], s.ev[1]);
   S.ms_contract = ms_contract;
   S.ms_merge = ms_merge;
   S.ms_fallback = ms_fallback;
-  S.ms_total = elapsed(c->ev[0], c->ev[5]);
+  S.ms_total = elapsed(s.ev[0], s.ev[5]);
   c->stats = S;
-  return k_out;
+  return q.k_out;
 }
 
 }  // namespace
+}  // namespace mdsctk
+
+using namespace mdsctk;
 
 extern "C" {
 
@@ -467,15 +531,10 @@ int mdsctk_knn_create(mdsctk_knn_ctx** out, int device, int precision) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     c->n_cu = prop.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
-    return MDSCTK_ERR_HIP;
-  }
-  for (auto& e : c->ev)
-    if (hipEventCreate(&e) != hipSuccess) { mdsctk_knn_destroy(c); return MDSCTK_ERR_HIP; }
-  if (dalloc(c, (void**)&c->gmax, 256, "scratch") || dalloc(c, (void**)&c->n_cand, 256, "scratch")) {
+  const int rc = slot_init(c, c->main);
+  if (rc) {
     mdsctk_knn_destroy(c);
-    return MDSCTK_ERR_NOMEM;
+    return rc;
   }
   *out = c;
   return 0;
@@ -484,12 +543,10 @@ int mdsctk_knn_create(mdsctk_knn_ctx** out, int device, int precision) {
 int mdsctk_knn_destroy(mdsctk_knn_ctx* c) {
   if (!c) return MDSCTK_ERR_ARG;
   (void)hipSetDevice(c->device);
-  if (c->st) (void)hipStreamSynchronize(c->st);
+  server_stop(c);
+  slot_release(c, c->main);
   for (auto& kv : c->allocs) (void)hipFree(kv.first);
   c->allocs.clear();
-  for (auto& e : c->ev)
-    if (e) (void)hipEventDestroy(e);
-  if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return 0;
 }
@@ -513,6 +570,17 @@ int mdsctk_knn_set_option(mdsctk_knn_ctx* c, int option, double value) {
     case MDSCTK_OPT_REF_SPLITS:
       if (value < 0) return fail(c, MDSCTK_ERR_ARG, "phases must be >= 0");
       c->forced_phases = (int)value;
+      return 0;
+    case MDSCTK_OPT_SERVE_SLOTS:
+      if (!(value >= 1 && value <= 4)) return fail(c, MDSCTK_ERR_ARG, "serve slots must be 1..4");
+      if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");
+      c->serve_slots = (int)value;
+      return 0;
+    case MDSCTK_OPT_SERVE_MAX_FRAMES:
+      if (!(value >= 1 && value <= (double)INT32_MAX))
+        return fail(c, MDSCTK_ERR_ARG, "serve max frames must be >= 1");
+      if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");
+      c->serve_max_frames = (int64_t)value;
       return 0;
     default:
       return fail(c, MDSCTK_ERR_ARG, "unknown option %d", option);
@@ -546,14 +614,22 @@ int mdsctk_knn_debug_scores(mdsctk_knn_ctx* c, const float* xyz_nm, int64_t n_fi
   if (!c) return MDSCTK_ERR_ARG;
   if (!c->has_ref) return fail(c, MDSCTK_ERR_STATE, "debug_scores: no reference set");
   if (c->prec == kExact) return fail(c, MDSCTK_ERR_STATE, "debug_scores: no contraction form");
+  if (!approx_s) return fail(c, MDSCTK_ERR_ARG, "debug_scores: null output");
   if (xyz_nm == nullptr) n_fit = c->ref.n;
   if (n_fit <= 0 || (double)n_fit * c->ref.n > 6.0e7)
     return fail(c, MDSCTK_ERR_ARG, "debug_scores: n_fit*n_ref must be in (0, 6e7]");
   HIP_TRY(c, hipSetDevice(c->device));
+  Slot& s = c->main;
   const bool nofit = flags & MDSCTK_KNN_NOFIT;
-  DevFrames* fit = nullptr;
-  int rc = stage_queries(c, xyz_nm, false, n_fit, &fit);
-  if (rc) return rc;
+  const DevFrames* fit = &c->ref;
+  int rc;
+  if (xyz_nm) {
+    if ((rc = slot_frames(c, s, n_fit))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(s.fit.xyz, xyz_nm, (size_t)n_fit * c->ref.natoms * 3 * sizeof(float),
+                              hipMemcpyHostToDevice, s.st));
+    if ((rc = prep_slot(c, s))) return rc;
+    fit = &s.fit;
+  }
   const size_t n = (size_t)n_fit * c->ref.n;
   void* buf = nullptr;
   if ((rc = dalloc(c, &buf, n * sizeof(double), "debug scores"))) return rc;
@@ -561,18 +637,16 @@ int mdsctk_knn_debug_scores(mdsctk_knn_ctx* c, const float* xyz_nm, int64_t n_fi
   L.nl = 4;
   L.cap = 256;
   hipError_t e = launch_contract(c->ref, *fit, L, nullptr, 1, 1, nofit, (double*)buf,
-                                 std::max(8, c->n_cu & ~7), c->st);  // any grid is valid
+                                 std::max(8, c->n_cu & ~7), s.st);  // any grid is valid
   if (e == hipSuccess)
-    e = hipMemcpyAsync(approx_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    e = hipMemcpyAsync(approx_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, s.st);
   if (e == hipSuccess && exact_s) {
-    std::vector<int> rows(n_fit);
-    for (int64_t i = 0; i < n_fit; ++i) rows[i] = (int)i;
     e = launch_exact_rows(c->ref, *fit, c->w, nullptr, (int)n_fit, nofit, (double*)buf, nullptr, 0,
-                          c->st);
+                          s.st);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(exact_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, c->st);
+      e = hipMemcpyAsync(exact_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, s.st);
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(s.st);
   dfree(c, buf);
   if (e != hipSuccess) return hip_fail(c, e, "debug_scores");
   return 0;
@@ -592,5 +666,28 @@ int mdsctk_knn_row_width(const mdsctk_knn_ctx* c, int32_t k, uint32_t flags) {
   if (flags & MDSCTK_KNN_KEEP_RANK0) return (int)std::min<int64_t>(k, c->ref.n);
   return (int)std::min<int64_t>(k, c->ref.n - 1);
 }
+
+int mdsctk_knn_submit(mdsctk_knn_ctx* c, const float* xyz_nm, int64_t n_fit, int32_t k,
+                      uint32_t flags, double* out_dist, int32_t* out_idx, int64_t* request_id) {
+  if (!c) return MDSCTK_ERR_ARG;
+  return server_submit(c, xyz_nm, n_fit, k, flags, out_dist, out_idx, request_id);
+}
+
+int mdsctk_knn_poll(mdsctk_knn_ctx* c, int64_t request_id, mdsctk_knn_request_info* info) {
+  if (!c) return MDSCTK_ERR_ARG;
+  return server_poll(c, request_id, info, false);
+}
+
+int mdsctk_knn_wait(mdsctk_knn_ctx* c, int64_t request_id, mdsctk_knn_request_info* info) {
+  if (!c) return MDSCTK_ERR_ARG;
+  return server_poll(c, request_id, info, true);
+}
+
+int mdsctk_knn_serve_stats_get(mdsctk_knn_ctx* c, mdsctk_knn_serve_stats* out) {
+  if (!c || !out) return MDSCTK_ERR_ARG;
+  return server_stats(c, out);
+}
+
+double mdsctk_knn_clock_ms(const mdsctk_knn_ctx* c) { return c ? clock_ms(c) : 0.0; }
 
 }  // extern "C"

@@ -34,6 +34,7 @@ struct DevFrames {
   int64_t n = 0;        // real frames
   int natoms = 0;
   int n_tiles = 0;      // padded tile count (multiple of kTileAlign)
+  int cap_tiles = 0;    // tiles the derived arrays are allocated for (>= n_tiles)
   int n_groups = 0;     // operand K-groups
   int prec = kFP32;
   float* xyz = nullptr;   // [n][natoms][3] as given (nm)

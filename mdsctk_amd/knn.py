@@ -27,6 +27,8 @@ KNN_KEEP_RANK0 = 4
 OPT_HBM_BUDGET = 1
 OPT_EPS_SCALE = 2
 OPT_REF_SPLITS = 3
+OPT_SERVE_SLOTS = 4
+OPT_SERVE_MAX_FRAMES = 5
 
 ERR_ARG = -1
 ERR_HIP = -2
@@ -51,6 +53,38 @@ class Stats(ctypes.Structure):
         ("ref_splits", ctypes.c_int32),
         ("list_capacity", ctypes.c_int32),
         ("eps_rel", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("n_batches", ctypes.c_int32),
+        ("n_fallback_rows", ctypes.c_int64),
+        ("t_submit_ms", ctypes.c_double),
+        ("t_admit_ms", ctypes.c_double),
+        ("t_done_ms", ctypes.c_double),
+        ("batch_frames", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("requests_done", ctypes.c_int64),
+        ("batches", ctypes.c_int64),
+        ("frames", ctypes.c_int64),
+        ("pairs", ctypes.c_int64),
+        ("max_batch_frames", ctypes.c_int64),
+        ("fallback_rows", ctypes.c_int64),
+        ("slots", ctypes.c_int32),
+        ("slot_frames", ctypes.c_int32),
+        ("busy_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -97,6 +131,16 @@ def load():
     L.mdsctk_knn_get_stats.restype = ctypes.c_int
     L.mdsctk_knn_row_width.argtypes = [vp, i32, u32]
     L.mdsctk_knn_row_width.restype = ctypes.c_int
+    L.mdsctk_knn_submit.argtypes = [vp, vp, i64, i32, u32, vp, vp, ctypes.POINTER(i64)]
+    L.mdsctk_knn_submit.restype = ctypes.c_int
+    L.mdsctk_knn_poll.argtypes = [vp, i64, ctypes.POINTER(RequestInfo)]
+    L.mdsctk_knn_poll.restype = ctypes.c_int
+    L.mdsctk_knn_wait.argtypes = [vp, i64, ctypes.POINTER(RequestInfo)]
+    L.mdsctk_knn_wait.restype = ctypes.c_int
+    L.mdsctk_knn_serve_stats_get.argtypes = [vp, ctypes.POINTER(ServeStats)]
+    L.mdsctk_knn_serve_stats_get.restype = ctypes.c_int
+    L.mdsctk_knn_clock_ms.argtypes = [vp]
+    L.mdsctk_knn_clock_ms.restype = dbl
     _bind_io(L)
     _lib = L
     return L
@@ -225,3 +269,66 @@ class KnnContext:
         s = Stats()
         self._check(self._L.mdsctk_knn_get_stats(self._h, ctypes.byref(s)))
         return s.as_dict()
+
+    # ---- serving (asynchronous requests, continuous batching)
+
+    def submit(self, xyz, k, flags=0) -> "Request":
+        """Queue a query; the returned Request owns the input copy and the
+        output arrays until it is done."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        assert xyz.ndim == 3 and xyz.shape[1] == self.natoms and xyz.shape[2] == 3
+        n_fit = xyz.shape[0]
+        width = self.row_width(k, flags)
+        d = np.zeros((n_fit, width), np.float64)
+        i = np.zeros((n_fit, width), np.int32)
+        rid = ctypes.c_int64()
+        self._check(self._L.mdsctk_knn_submit(self._h, _ptr(xyz), n_fit, k, flags, _ptr(d), _ptr(i),
+                                              ctypes.byref(rid)))
+        return Request(rid.value, xyz, d, i)
+
+    def _finish(self, req, rc, info):
+        req.info = info.as_dict()
+        if rc < 0:
+            req.error = KnnError(rc, self._L.mdsctk_knn_last_error(self._h).decode())
+            raise req.error
+        req.done = True
+        return req
+
+    def poll(self, req) -> bool:
+        """True when `req` is done (its outputs are final); raises on failure."""
+        if req.done:
+            return True
+        info = RequestInfo()
+        rc = self._L.mdsctk_knn_poll(self._h, req.id, ctypes.byref(info))
+        if rc == ERR_PENDING:
+            return False
+        self._finish(req, rc, info)
+        return True
+
+    def wait(self, req) -> "Request":
+        if req.done:
+            return req
+        info = RequestInfo()
+        rc = self._L.mdsctk_knn_wait(self._h, req.id, ctypes.byref(info))
+        return self._finish(req, rc, info)
+
+    def serve_stats(self) -> dict:
+        s = ServeStats()
+        self._check(self._L.mdsctk_knn_serve_stats_get(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def clock_ms(self) -> float:
+        return self._L.mdsctk_knn_clock_ms(self._h)
+
+
+class Request:
+    """A submitted query: `dist`/`idx` are final once `done` is True."""
+
+    def __init__(self, rid, xyz, dist, idx):
+        self.id = rid
+        self.xyz = xyz
+        self.dist = dist
+        self.idx = idx
+        self.done = False
+        self.error = None
+        self.info = None
