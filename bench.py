@@ -66,6 +66,11 @@ class Hip:
         if rc != 0:
             raise RuntimeError(f"hipMemcpy failed: {rc}")
 
+    def copy_d2d(self, dst, src, n):
+        rc = self.L.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(n), 3)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+
     def sync(self):
         self.L.hipDeviceSynchronize()
 
@@ -208,7 +213,20 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=128, help="fit rows of the CPU baseline sample (0 = skip)")
     ap.add_argument("--check-rows", type=int, default=8, help="rows checked against the FP64 oracle")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device (HIP blit kernel) 3 times and exit: the "
+                         "plain copy of known size that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        hip = Hip()
+        a, b = hip.malloc(args.copy_probe), hip.malloc(args.copy_probe)
+        for _ in range(3):
+            hip.copy_d2d(b, a, args.copy_probe)
+        hip.sync()
+        hip.free(a)
+        hip.free(b)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the knn_rms engine runs on a single "

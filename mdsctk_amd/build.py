@@ -23,6 +23,8 @@ BIN_DIR = os.path.join(HERE, "bin")
 OBJ_DIR = os.path.join(ROOT, "_build", "obj")
 LIB_PATH = os.path.join(LIB_DIR, "libmdsctk_knn.so")
 CLI_PATH = os.path.join(BIN_DIR, "knn_rms")
+MAKE_SYSPARSE_PATH = os.path.join(BIN_DIR, "make_sysparse")
+DECOMP_SPARSE_PATH = os.path.join(BIN_DIR, "decomp_sparse")
 
 ARCH = os.environ.get("MDSCTK_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -32,7 +34,10 @@ DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
           f"-I{os.path.join(ROOT, 'include')}"]
 
-LIB_SOURCES = ["knn_kernels.hip", "api.cpp", "serve.cpp", "xtc.cpp", "topology.cpp"]
+LIB_SOURCES = ["knn_kernels.hip", "api.cpp", "serve.cpp", "xtc.cpp", "topology.cpp", "spectral.cpp"]
+# executables: (name, main source)
+TOOLS = [("knn_rms", "knn_rms_main.cpp"), ("make_sysparse", "make_sysparse_main.cpp"),
+         ("decomp_sparse", "decomp_sparse_main.cpp")]
 
 
 def _run(cmd, verbose):
@@ -66,13 +71,18 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     tmp = LIB_PATH + ".tmp"
     _run([HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB_PATH)
-    cli_src = os.path.join(CSRC, "knn_rms_main.cpp")
-    if os.path.exists(cli_src):
-        os.makedirs(BIN_DIR, exist_ok=True)
-        tmp = CLI_PATH + ".tmp"
-        _run([HIPCC] + COMMON + [cli_src, "-o", tmp, f"-L{LIB_DIR}", "-lmdsctk_knn",
-                                  "-Wl,-rpath,$ORIGIN/../lib"], verbose)
-        os.replace(tmp, CLI_PATH)
+    os.makedirs(BIN_DIR, exist_ok=True)
+
+    def link_tool(tool):
+        name, main = tool
+        exe = os.path.join(BIN_DIR, name)
+        tmp = exe + ".tmp"
+        _run([HIPCC] + COMMON + [os.path.join(CSRC, main), "-o", tmp, f"-L{LIB_DIR}",
+                                  "-lmdsctk_knn", "-Wl,-rpath,$ORIGIN/../lib"], verbose)
+        os.replace(tmp, exe)
+
+    with cf.ThreadPoolExecutor(max_workers=len(TOOLS)) as ex:
+        list(ex.map(link_tool, TOOLS))
     return LIB_PATH
 
 

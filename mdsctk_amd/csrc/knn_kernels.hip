@@ -188,7 +188,9 @@ __device__ __forceinline__ double qcp_score_approx(const float* m, double ga, do
   if (nofit) return ga + gb - 2.0 * ((double)m[0] + (double)m[4] + (double)m[8]);
   const double g = 0.5 * (ga + gb);
   if (!(g > 0.0)) return 0.0;
-  const double ig = 1.0 / g;
+  // Any consistent scale works here: s = 2 g (1 - l) is evaluated with the
+  // same g, so the reciprocal's last-bit error only rescales l.
+  const double ig = __builtin_amdgcn_rcp(g);
   double S[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) S[i] = (double)m[i] * ig;
@@ -428,6 +430,7 @@ struct ContractParams {
   int* cnt;
   float* ovf;
   unsigned* thr;
+  unsigned* sync;
   int nl, cap;
   int nofit;
   double* dbg;
@@ -575,17 +578,22 @@ __device__ __attribute__((noinline)) void compact_lists(uint2* __restrict__ ent,
 // barrier (s_barrier itself is not a memory fence).
 template <int N>
 __device__ __forceinline__ void stage_barrier() {
-  static_assert(N >= 0 && N < 16, "vmcnt range");
+  static_assert(N >= 0 && N < 64, "vmcnt range");
   asm volatile("" ::: "memory");
-  // s_waitcnt: vmcnt = N, expcnt = 7 (no wait), lgkmcnt = 0
-  __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4));
+  // s_waitcnt: vmcnt = N (bits 3:0 and 15:14), expcnt = 7 (no wait),
+  // lgkmcnt = 0
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4));
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
 }
 
+// Polls (each followed by a ~128-cycle sleep) a workgroup spends waiting for
+// its group at a tile boundary before it stops waiting.
+constexpr int kSyncSpinMax = 500;
+
 // FT = fit tiles per workgroup (2 or 4); the workgroup has 2*FT waves, one
 // per (reference tile, fit tile) pair.
-template <int PREC, int PER, int FT>
+template <int PREC, int PER, int FT, bool DBG>
 __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(ContractParams P) {
   constexpr int BPG = (PREC == kFP32) ? 3 : 6;
   constexpr int TILE_VEC = BPG * 64;
@@ -631,11 +639,59 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
   const int logical = (int)(blockIdx.x & 7) * (nwg >> 3) + (int)(blockIdx.x >> 3);
   const int n_items = fit_blocks * P.phases;
 
-  for (int item = logical; item < n_items; item += nwg) {
+  // Keeping step: the workgroups of one XCD read the same fit and reference
+  // tiles at about the same time only while they progress together, and
+  // small speed differences add up over an item's hundreds of reference
+  // tiles until the L2 no longer holds what a slower workgroup needs.  So
+  // after every reference tile each workgroup arrives at its XCD's counter
+  // and waits (bounded) for the rest of the group.  Every workgroup arrives
+  // exactly rounds * t_max times: tiles an item lacks, and rounds a
+  // workgroup has no item for, arrive without work.  The wait only affects
+  // speed: a workgroup whose wait times out stops waiting for the rest of
+  // the launch (e.g. when not all of the grid is resident at once).
+  const int rounds = (n_items + nwg - 1) / nwg;
+  const int t_max = (P.ref_pairs + P.phases - 1) / P.phases;
+  const unsigned group_size = (unsigned)(nwg >> 3);
+  unsigned* const grp_cnt =
+      P.sync ? P.sync + (logical / (int)group_size) * kSyncStride : nullptr;
+  const bool sync_lane = (tid == (int)blockDim.x - 64);  // lane 0 of the last wave
+  unsigned arrivals = 0;
+  bool wait_ok = true;
+  auto keep_step = [&]() {
+    if (!grp_cnt) return;
+    ++arrivals;
+    if (sync_lane) {
+      __hip_atomic_fetch_add(grp_cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (wait_ok) {
+        const unsigned target = group_size * arrivals;
+        for (int spin = 0;; ++spin) {
+          if (__hip_atomic_load(grp_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+          if (spin >= kSyncSpinMax) {
+            wait_ok = false;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+    }
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+
+  for (int round = 0; round < rounds; ++round) {
+    const int item = logical + round * nwg;
+    if (item >= n_items) {
+      for (int t = 0; t < t_max; ++t) keep_step();
+      continue;
+    }
     const int fb = item / P.phases;
     const int ph = item - fb * P.phases;
     const int n_tiles = (P.ref_pairs - ph + P.phases - 1) / P.phases;  // ref pairs ph, ph+R, ...
-    if (n_tiles <= 0) continue;
+    if (n_tiles <= 0) {
+      for (int t = 0; t < t_max; ++t) keep_step();
+      continue;
+    }
     const int64_t fit_idx = (int64_t)(FT * fb + fi) * kTile + (lane & 31);
     const bool fvalid = fit_idx < P.n_fit;
     const int list_id = (ph * 2 + ri) * 2 + (lane >> 5);
@@ -717,7 +773,7 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
 #pragma unroll
         for (int q = 0; q < 9; ++q) m[q] = acc[q][r];
         const double gr = gr_tile[ri * kTile + rl];
-        if (P.dbg) {
+        if constexpr (DBG) {
           P.dbg[fit_idx * P.n_ref + ref_idx] = qcp_score_approx(m, gr, gf, INFINITY, P.nofit);
           continue;
         }
@@ -728,13 +784,15 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
           ++cnt;
         }
       }
-      if (P.dbg) continue;
+      if constexpr (DBG) continue;
       const float thr_before = thr;
       compact_lists<PER>(P.ent, my_base, cnt, thr, ovf, e2, P.k1, P.cap, fvalid);
       thr = fminf(thr, __shfl_xor(thr, 32));
       if (fvalid && P.thr && thr < thr_before) atomicMin(&P.thr[fit_idx], __float_as_uint(thr));
+      keep_step();
     }
-    if (fvalid && !P.dbg) {
+    for (int t = n_tiles; t < t_max; ++t) keep_step();
+    if (fvalid && !DBG) {
       P.cnt[hdr] = cnt;
       P.ovf[hdr] = ovf;
     }
@@ -1063,14 +1121,15 @@ __global__ __launch_bounds__(256) void select_rows_kernel(const double* __restri
   }
 }
 
-__global__ void init_lists_kernel(int* cnt, float* ovf, unsigned* thr, int64_t n_hdr,
-                                  int64_t n_fit) {
+__global__ void init_lists_kernel(int* cnt, float* ovf, unsigned* thr, unsigned* sync,
+                                  int64_t n_hdr, int64_t n_fit) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_hdr) {
     cnt[i] = 0;
     ovf[i] = INFINITY;
   }
   if (i < n_fit) thr[i] = kInfBits;
+  if (sync && i < kSyncGroups * kSyncStride) sync[i] = 0u;
 }
 
 __global__ void max_kernel(const double* __restrict__ v, int64_t n, double* out) {
@@ -1134,9 +1193,10 @@ hipError_t launch_prep(const DevFrames& f, const DevWeights& w, hipStream_t st) 
 
 hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st) {
   const int64_t n_hdr = n_fit * L.nl;
-  const int64_t n = n_hdr > n_fit ? n_hdr : n_fit;
+  int64_t n = n_hdr > n_fit ? n_hdr : n_fit;
+  if (n < kSyncGroups * kSyncStride) n = kSyncGroups * kSyncStride;
   hipLaunchKernelGGL(init_lists_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     L.cnt, L.ovf, L.thr, n_hdr, n_fit);
+                     L.cnt, L.ovf, L.thr, L.sync, n_hdr, n_fit);
   return hipGetLastError();
 }
 
@@ -1175,6 +1235,7 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   P.cnt = L.cnt;
   P.ovf = L.ovf;
   P.thr = L.thr;
+  P.sync = dbg_scores ? nullptr : L.sync;
   P.nl = L.nl;
   P.cap = L.cap;
   P.nofit = nofit ? 1 : 0;
@@ -1183,16 +1244,19 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   // FP32 runs two 4-wave workgroups per CU (2 x 2 tiles each); the split
   // forms one 8-wave workgroup per CU (2 x 4 tiles: a third less L2 traffic
   // per MFMA).
-#define MDSCTK_CONTRACT(PR, PER, FT) \
-  hipLaunchKernelGGL((contract_kernel<PR, PER, FT>), grid, dim3(64 * kRefTilesPerWG * FT), 0, st, P)
+#define MDSCTK_CONTRACT(PR, PER, FT, DBG)                                                     \
+  hipLaunchKernelGGL((contract_kernel<PR, PER, FT, DBG>), grid, dim3(64 * kRefTilesPerWG * FT), 0, \
+                     st, P)
 #define MDSCTK_CONTRACT_CAP(PR, FT)         \
   do {                                      \
-    if (L.cap == 256)                       \
-      MDSCTK_CONTRACT(PR, 4, FT);           \
+    if (dbg_scores)                         \
+      MDSCTK_CONTRACT(PR, 4, FT, true);     \
+    else if (L.cap == 256)                  \
+      MDSCTK_CONTRACT(PR, 4, FT, false);    \
     else if (L.cap == 512)                  \
-      MDSCTK_CONTRACT(PR, 8, FT);           \
+      MDSCTK_CONTRACT(PR, 8, FT, false);    \
     else if (L.cap == 1024)                 \
-      MDSCTK_CONTRACT(PR, 16, FT);          \
+      MDSCTK_CONTRACT(PR, 16, FT, false);   \
     else                                    \
       return hipErrorInvalidValue;          \
   } while (0)

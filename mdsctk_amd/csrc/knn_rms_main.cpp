@@ -25,19 +25,18 @@
 #include <thread>
 #include <vector>
 
+#include "cli_options.h"
 #include "mdsctk_io.h"
 #include "mdsctk_knn.h"
 
 namespace {
 
-struct Opt {
-  const char* lng;
-  char shrt;
-  bool takes_value;
-  const char* help;
-};
+using mdsctk_cli::Opt;
+using mdsctk_cli::parse_bool;
+using mdsctk_cli::parse_int;
+using mdsctk_cli::write_all;
 
-const Opt kOpts[] = {
+const std::vector<Opt> kOpts = {
     {"help", 'h', false, "show this help message and exit"},
     {"threads", 't', true, "Input:  Number of threads to start (int)"},
     {"knn", 'k', true, "Input:  K-nearest neighbors (int)"},
@@ -53,92 +52,10 @@ const Opt kOpts[] = {
     {"device", 0, true, "Input:  HIP device index (int)"},
 };
 
-void usage() {
-  std::printf("usage: knn_rms [options]\nProgram options:\n");
-  for (const auto& o : kOpts) {
-    char left[64];
-    if (o.shrt)
-      std::snprintf(left, sizeof left, "  -%c [ --%s ]%s", o.shrt, o.lng, o.takes_value ? " arg" : "");
-    else
-      std::snprintf(left, sizeof left, "  --%s%s", o.lng, o.takes_value ? " arg" : "");
-    std::printf("%-36s %s\n", left, o.help);
-  }
-}
+void usage() { mdsctk_cli::usage("knn_rms", kOpts); }
 
-bool parse_bool(const std::string& v, bool* out) {
-  std::string s;
-  for (char c : v) s.push_back((char)std::tolower((unsigned char)c));
-  if (s == "true" || s == "1" || s == "yes" || s == "on") { *out = true; return true; }
-  if (s == "false" || s == "0" || s == "no" || s == "off") { *out = false; return true; }
-  return false;
-}
-
-bool parse_int(const std::string& v, int* out) {
-  char* end = nullptr;
-  long x = std::strtol(v.c_str(), &end, 10);
-  if (v.empty() || *end) return false;
-  *out = (int)x;
-  return true;
-}
-
-// Command-line parsing in the style of boost::program_options' defaults:
-// "-k 30", "-k30", "--knn 30", "--knn=30", and unambiguous prefixes of long
-// names.  Returns false (with a message) on an unknown or incomplete option.
 bool parse(int argc, char** argv, std::map<std::string, std::string>& vals, std::string& err) {
-  for (int a = 1; a < argc; ++a) {
-    std::string arg = argv[a];
-    const Opt* opt = nullptr;
-    std::string value;
-    bool have_value = false;
-    if (arg.rfind("--", 0) == 0) {
-      std::string name = arg.substr(2);
-      const size_t eq = name.find('=');
-      if (eq != std::string::npos) {
-        value = name.substr(eq + 1);
-        name = name.substr(0, eq);
-        have_value = true;
-      }
-      int matches = 0;
-      for (const auto& o : kOpts) {
-        if (name == o.lng) { opt = &o; matches = 1; break; }
-        if (std::string(o.lng).rfind(name, 0) == 0) { opt = &o; ++matches; }
-      }
-      if (!opt || matches != 1) {
-        err = "unrecognised option '" + arg + "'";
-        return false;
-      }
-    } else if (arg.size() >= 2 && arg[0] == '-') {
-      for (const auto& o : kOpts)
-        if (o.shrt == arg[1]) opt = &o;
-      if (!opt) {
-        err = "unrecognised option '" + arg + "'";
-        return false;
-      }
-      if (arg.size() > 2) {
-        value = arg.substr(2);
-        have_value = true;
-      }
-    } else {
-      err = "unexpected argument '" + arg + "'";
-      return false;
-    }
-    if (opt->takes_value && !have_value) {
-      if (a + 1 >= argc) {
-        err = std::string("the required argument for option '--") + opt->lng + "' is missing";
-        return false;
-      }
-      value = argv[++a];
-    }
-    vals[opt->lng] = value;
-  }
-  return true;
-}
-
-bool write_all(const std::string& path, const void* data, size_t bytes) {
-  FILE* f = std::fopen(path.c_str(), "wb");
-  if (!f) return false;
-  const bool ok = bytes == 0 || std::fwrite(data, 1, bytes, f) == bytes;
-  return std::fclose(f) == 0 && ok;
+  return mdsctk_cli::parse(argc, argv, kOpts, vals, err);
 }
 
 }  // namespace

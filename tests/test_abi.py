@@ -15,7 +15,7 @@ def declared_functions():
     names = set()
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         text = open(h).read()
-        names.update(re.findall(r"^\s*(?:const\s+char\s*\*|int|void)\s+(mdsctk_\w+)\s*\(", text, re.M))
+        names.update(re.findall(r"^\s*(?:const\s+char\s*\*|int|void|double)\s+(mdsctk_\w+)\s*\(", text, re.M))
     return sorted(names)
 
 
