@@ -193,6 +193,92 @@ def run_serving(knn, synth, oracle, prec, args):
             "mode": "host buffers in and out (PCIe included), latency from scheduled arrival"}
 
 
+def run_mixed_sets(knn, synth, oracle, args):
+    """Mixed-size batched serving: three resident sets (30,000 x 150,
+    100,000 x 500, 20,000 x 2,000 atoms), Poisson arrivals of 1..2048-frame
+    queries (log-uniform) to a random set, k = 32, split-BF16; one context
+    (serving thread, batch slots) per set.  Then the 30,000-frame set's
+    self-kNN graph (make_sysparse semantics) and the 10 leading eigenvalues
+    of its normalised Gaussian affinity (sigma = median kNN distance) against
+    the CPU pipeline: FP64 oracle kNN, edge-dictionary make_sysparse, ARPACK."""
+    import oracle.graph as og
+    from mdsctk_amd import graph
+    sets = [(30_000, 150), (100_000, 500), (20_000, 2_000)]
+    kq, pool_n = 32, 2048
+    rng = np.random.default_rng(args.seed + 21)
+    data, pools, w = [], [], []
+    for j, (n, na) in enumerate(sets):
+        data.append(synth.trajectory(n, na, args.seed + 30 + j))
+        pools.append(synth.trajectory(pool_n + 64, na, args.seed + 40 + j, base_seed=args.seed + 30 + j))
+        w.append(synth.masses(na))
+    rate, dur = args.mixed_rate, args.mixed_seconds
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, int(rate * dur * 1.5) + 10))
+    arrivals = arrivals[arrivals < dur]
+    target = rng.integers(0, len(sets), len(arrivals))
+    sizes = np.minimum(np.floor(np.exp(rng.uniform(0.0, np.log(2049.0), len(arrivals)))), 2048).astype(np.int64)
+    starts = rng.integers(0, 64, len(arrivals))
+    ctxs = [knn.KnnContext(0, "bf16") for _ in sets]
+    lat = [[] for _ in sets]
+    pairs = 0.0
+    try:
+        for c, x, ww in zip(ctxs, data, w):
+            c.set_reference(x, ww)
+        for c, pool in zip(ctxs, pools):       # warm-up: sizes the batch slots
+            c.wait(c.submit(pool[:2048], kq))
+            c.wait(c.submit(pool[:1], kq))
+        t0 = ctxs[0].clock_ms() + 10.0
+        outstanding = []
+        for a, tg, nq, s0 in zip(arrivals, target, sizes, starts):
+            tgt = t0 + a * 1e3
+            while ctxs[0].clock_ms() < tgt:
+                still = []
+                for (j, r, t_arr) in outstanding:
+                    if ctxs[j].poll(r):
+                        lat[j].append(r.info["t_done_ms"] - t_arr)
+                    else:
+                        still.append((j, r, t_arr))
+                outstanding = still
+                if tgt - ctxs[0].clock_ms() > 1.5:
+                    time.sleep(0.0005)
+            r = ctxs[tg].submit(pools[tg][s0:s0 + nq], kq)
+            outstanding.append((int(tg), r, tgt))
+            pairs += float(nq) * sets[tg][0]
+        for (j, r, t_arr) in outstanding:
+            ctxs[j].wait(r)
+            lat[j].append(r.info["t_done_ms"] - t_arr)
+        # graph check on the 30,000-frame set
+        t = time.time()
+        d, i = ctxs[0].query(None, kq)
+        gpu_s = time.time() - t
+    finally:
+        for c in ctxs:
+            c.close()
+    m = graph.ssm_build(d, i)
+    sigma = float(np.median(d))
+    ev, _, res, got = graph.spectral_decomp(m, sigma, 10, vectors=False)
+    t = time.time()
+    d_o, i_o = oracle.knn_exact(data[0], data[0], w[0], kq)
+    cpu_s = time.time() - t
+    n_o, pcol_o, irow_o, vals_o = og.ssm_build(d_o, i_o)
+    w_o, _ = og.spectral_decomp(n_o, pcol_o, irow_o, vals_o, float(np.median(d_o)), 10)
+    all_lat = np.concatenate([np.array(l) for l in lat if l])
+    per_set = {f"{n}x{na}": {"requests": len(l), "p50_ms": float(np.percentile(l, 50)),
+                              "p99_ms": float(np.percentile(l, 99))}
+               for (n, na), l in zip(sets, lat) if l}
+    return {"workload": f"3 resident sets {sets}, Poisson {rate:.0f} req/s over {dur:.0f} s, "
+                        "1..2048 frames log-uniform, random set, k=32, split-BF16",
+            "requests": int(len(all_lat)), "p50_ms": float(np.percentile(all_lat, 50)),
+            "p99_ms": float(np.percentile(all_lat, 99)), "per_set": per_set,
+            "pairs_per_s": pairs / (dur + 1e-3),
+            "graph_check": {"set": "30000 x 150 self kNN, k=32", "sigma": sigma,
+                            "eigenvalues": [float(v) for v in ev],
+                            "cpu_pipeline_eigenvalues": [float(v) for v in w_o],
+                            "max_abs_diff": float(np.abs(ev - w_o).max()), "tolerance": 1e-6,
+                            "converged": int(got), "max_residual": float(res.max()),
+                            "same_edges": bool((m.pcol == pcol_o).all() and (m.irow == irow_o).all()),
+                            "gpu_knn_s": gpu_s, "cpu_knn_s": cpu_s}}
+
+
 def mean_stat(stats, key):
     return float(np.mean([s[key] for s in stats]))
 
@@ -210,6 +296,10 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=30.0,
                     help="length of the Poisson serving run (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="mean request arrivals per second")
+    ap.add_argument("--mixed-seconds", type=float, default=0.0,
+                    help="run the three-set mixed serving workload and its graph check for this "
+                         "many seconds (0 = skip; the CPU side of the graph check takes minutes)")
+    ap.add_argument("--mixed-rate", type=float, default=100.0, help="mixed workload arrivals per second")
     ap.add_argument("--cpu-rows", type=int, default=128, help="fit rows of the CPU baseline sample (0 = skip)")
     ap.add_argument("--check-rows", type=int, default=8, help="rows checked against the FP64 oracle")
     ap.add_argument("--seed", type=int, default=2026)
@@ -306,6 +396,9 @@ def main():
 
     if args.serving_seconds > 0:
         out["serving"] = run_serving(knn, synth, oracle, prec, args)
+
+    if args.mixed_seconds > 0:
+        out["mixed_sets"] = run_mixed_sets(knn, synth, oracle, args)
 
     if args.cpu_rows > 0:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
