@@ -346,10 +346,8 @@ int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist
   int rc = ensure(c, &s.ws_lists, &s.ws_lists_bytes, ent_bytes, "candidate lists");
   if (rc) return rc;
   L.ent = (uint2*)s.ws_lists;
-  const size_t sync_bytes = kSyncGroups * kSyncStride * sizeof(unsigned);
   const size_t small = n_hdr * (sizeof(int) + sizeof(float)) +
-                       (size_t)n_fit * (sizeof(unsigned) + sizeof(float) + sizeof(int)) +
-                       sync_bytes + 256;
+                       (size_t)n_fit * (sizeof(unsigned) + sizeof(float) + sizeof(int)) + 256;
   rc = ensure(c, &s.ws_small, &s.ws_small_bytes, small, "list headers");
   if (rc) return rc;
   char* sp = (char*)s.ws_small;
@@ -357,8 +355,7 @@ int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist
   L.ovf = (float*)sp; sp += n_hdr * sizeof(float);
   L.thr = (unsigned*)sp; sp += n_fit * sizeof(unsigned);
   float* eps2 = (float*)sp; sp += n_fit * sizeof(float);
-  int* d_fb = (int*)sp; sp += n_fit * sizeof(int);
-  L.sync = (unsigned*)(((uintptr_t)sp + 127) & ~(uintptr_t)127);
+  int* d_fb = (int*)sp;
   const double kappa = kappa_for(c->prec, c->ref.natoms) * c->eps_scale;
   HIP_TRY(c, launch_init_lists(L, n_fit, s.st));
   HIP_TRY(c, launch_eps(fit, c->ref, kappa, eps2, s.gmax, s.st));

@@ -60,15 +60,7 @@ struct Lists {
   int* cnt = nullptr;         // [n_fit][nl]
   float* ovf = nullptr;       // [n_fit][nl] smallest key dropped on overflow
   unsigned* thr = nullptr;    // [n_fit] shared threshold (float bits)
-  // [kSyncGroups * kSyncStride] tile-boundary arrival counters of the
-  // contraction's workgroup groups (zeroed with the lists; nullptr = none)
-  unsigned* sync = nullptr;
 };
-
-// Workgroup groups that keep step through the reference tiles (one per XCD)
-// and the spacing of their counters (one 128-byte line each).
-constexpr int kSyncGroups = 8;
-constexpr int kSyncStride = 32;
 
 // Launchers (all asynchronous on `st`).
 hipError_t launch_prep(const DevFrames& f, const DevWeights& w, hipStream_t st);

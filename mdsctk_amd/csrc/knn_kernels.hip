@@ -213,61 +213,6 @@ __device__ __forceinline__ double qcp_score_approx(const float* m, double ga, do
   return (s == s && s < INFINITY) ? s : 0.0;
 }
 
-// Wave-level bitonic sort, ascending by (key, val), of 64*PER elements held
-// PER per lane with element index e = lane*PER + j.
-__device__ __forceinline__ bool kv_less(unsigned ka, int va, unsigned kb, int vb) {
-  return ka < kb || (ka == kb && va < vb);
-}
-
-template <int PER>
-__device__ __forceinline__ void wave_sort(unsigned (&key)[PER], int (&val)[PER]) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int size = 2; size <= 64 * PER; size <<= 1) {
-#pragma unroll
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      if (stride >= PER) {
-        const int ls = stride / PER;
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-          const int e = lane * PER + j;
-          const unsigned ok = __shfl_xor(key[j], ls);
-          const int ov = __shfl_xor(val[j], ls);
-          const bool up = (e & size) == 0;
-          const bool lower = (e & stride) == 0;
-          const bool take_other = (lower == up) ? kv_less(ok, ov, key[j], val[j])
-                                                : kv_less(key[j], val[j], ok, ov);
-          if (take_other) { key[j] = ok; val[j] = ov; }
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-          const int jp = j ^ stride;
-          if (jp > j) {
-            const int e = lane * PER + j;
-            const bool up = (e & size) == 0;
-            const bool sw = up ? kv_less(key[jp], val[jp], key[j], val[j])
-                               : kv_less(key[j], val[j], key[jp], val[jp]);
-            if (sw) {
-              const unsigned tk = key[j]; key[j] = key[jp]; key[jp] = tk;
-              const int tv = val[j]; val[j] = val[jp]; val[jp] = tv;
-            }
-          }
-        }
-      }
-    }
-  }
-}
-
-template <int PER>
-__device__ __forceinline__ unsigned pick(const unsigned (&a)[PER], int j) {
-  unsigned r = a[0];
-#pragma unroll
-  for (int i = 1; i < PER; ++i)
-    if (i == j) r = a[i];
-  return r;
-}
-
 // ---------------------------------------------------------------------------
 // prep: one 192-thread workgroup per 32-frame tile.  Thread t -> plane
 // c = t/64, lane l = t%64 of that plane's operand block.
@@ -430,7 +375,6 @@ struct ContractParams {
   int* cnt;
   float* ovf;
   unsigned* thr;
-  unsigned* sync;
   int nl, cap;
   int nofit;
   double* dbg;
@@ -507,10 +451,37 @@ __device__ __forceinline__ void dma_group(const uint4* __restrict__ src_tile, in
                                      16, 0, 0);
 }
 
-// Compact the lists of every lane whose list is nearly full.  Each list is
-// sorted by the whole wave, its (k1)-th smallest key tau found, and only
-// entries <= tau + 2 eps kept (at most cap-64 of them; the smallest dropped
-// key is remembered in `ovf`).
+// Number of keys below `t` over the wave's PER x 64 slots.
+template <int PER>
+__device__ __forceinline__ int wave_count_below(const unsigned (&key)[PER], unsigned t) {
+  int c = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) c += __popcll(__ballot(key[j] < t));
+  return c;
+}
+
+// Smallest value t of the form (prefix << low) | ((1 << low) - 1) such that
+// more than `rank` keys are <= t: with low = 0 the rank-th smallest key
+// (0-based) exactly, with low > 0 an upper bound within 2^low units of it.
+// Bitwise search from the top bit, one ballot round per bit.
+template <int PER>
+__device__ __forceinline__ unsigned wave_select(const unsigned (&key)[PER], int rank, int low) {
+  unsigned t = 0;
+  for (int bit = 31; bit >= low; --bit) {
+    const unsigned cand = t | (1u << bit);
+    if (wave_count_below<PER>(key, cand) <= rank) t = cand;
+  }
+  return low ? (t | ((1u << low) - 1u)) : t;
+}
+
+// Compact the lists of every lane whose list is nearly full.  The whole wave
+// takes each such list in turn: it finds (an upper bound, within 2^-7
+// relative, of) the list's k1-th smallest key tau, keeps the entries
+// <= tau + 2 eps -- at most cap-64 of them; when more qualify, only those
+// below the (cap-64)-th smallest key d stay and d is remembered in `ovf` --
+// and lowers the lane's threshold to the cut.  Float keys are non-negative,
+// so their bit patterns order like the values.  Kept entries are written
+// back unsorted (the merge selects, it does not need order).
 template <int PER>
 __device__ __attribute__((noinline)) void compact_lists(uint2* __restrict__ ent, int64_t my_base, int& cnt,
                                               float& thr, float& ovf, float eps2, int k1,
@@ -524,44 +495,54 @@ __device__ __attribute__((noinline)) void compact_lists(uint2* __restrict__ ent,
     const int n = __shfl(cnt, L);
     const float e2 = __shfl(eps2, L);
     unsigned key[PER];
-    int val[PER];
+    unsigned val[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int e = lane * PER + j;
       if (e < n) {
         const uint2 x = ent[base + e];
         key[j] = x.x;
-        val[j] = (int)x.y;
+        val[j] = x.y;
       } else {
         key[j] = kInfBits;
-        val[j] = INT_MAX;
+        val[j] = 0u;
       }
     }
-    wave_sort<PER>(key, val);
-    const int tr = k1 - 1;
-    const unsigned tk = __shfl(pick<PER>(key, tr % PER), tr / PER);
-    const float tau = (n >= k1) ? __uint_as_float(tk) : INFINITY;
-    const float cut = tau * (1.0f + 2.4e-7f) + e2;
-    int mine = 0;
-#pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const int e = lane * PER + j;
-      mine += (e < n && __uint_as_float(key[j]) <= cut) ? 1 : 0;
+    float cut = INFINITY;
+    if (n >= k1) {
+      const float tau = __uint_as_float(wave_select<PER>(key, k1 - 1, 16));
+      cut = tau * (1.0f + 2.4e-7f) + e2;
     }
-    int keep = wave_sum_i(mine);
+    unsigned limit = __float_as_uint(cut);  // keep key <= limit
     const int kcap = cap - 64;
     float dropped = INFINITY;
+    int keep = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) keep += __popcll(__ballot(key[j] <= limit && key[j] != kInfBits));
     if (keep > kcap) {
-      dropped = __uint_as_float(__shfl(pick<PER>(key, kcap % PER), kcap / PER));
-      keep = kcap;
+      const unsigned d = wave_select<PER>(key, kcap, 0);  // the kcap-th smallest, exactly
+      dropped = __uint_as_float(d);
+      limit = d - 1u;                                     // keep key < d
     }
+    // Stream compaction in slot order e = lane*PER + j.
+    const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int pos = 0;
+    bool kp[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const int e = lane * PER + j;
-      if (e < keep) ent[base + e] = make_uint2(key[j], (unsigned)val[j]);
+      kp[j] = key[j] <= limit && key[j] != kInfBits;
+      pos += __popcll(__ballot(kp[j]) & lanes_below);
     }
+    int wrote = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (kp[j]) {
+        ent[base + pos + wrote] = make_uint2(key[j], val[j]);
+        ++wrote;
+      }
+    const int total = wave_sum_i(wrote);
     if (lane == L) {
-      cnt = keep;
+      cnt = total;
       thr = fminf(thr, cut);
       if (dropped < INFINITY) {
         ovf = fminf(ovf, dropped);
@@ -586,10 +567,6 @@ __device__ __forceinline__ void stage_barrier() {
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
 }
-
-// Polls (each followed by a ~128-cycle sleep) a workgroup spends waiting for
-// its group at a tile boundary before it stops waiting.
-constexpr int kSyncSpinMax = 500;
 
 // FT = fit tiles per workgroup (2 or 4); the workgroup has 2*FT waves, one
 // per (reference tile, fit tile) pair.
@@ -639,59 +616,14 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
   const int logical = (int)(blockIdx.x & 7) * (nwg >> 3) + (int)(blockIdx.x >> 3);
   const int n_items = fit_blocks * P.phases;
 
-  // Keeping step: the workgroups of one XCD read the same fit and reference
-  // tiles at about the same time only while they progress together, and
-  // small speed differences add up over an item's hundreds of reference
-  // tiles until the L2 no longer holds what a slower workgroup needs.  So
-  // after every reference tile each workgroup arrives at its XCD's counter
-  // and waits (bounded) for the rest of the group.  Every workgroup arrives
-  // exactly rounds * t_max times: tiles an item lacks, and rounds a
-  // workgroup has no item for, arrive without work.  The wait only affects
-  // speed: a workgroup whose wait times out stops waiting for the rest of
-  // the launch (e.g. when not all of the grid is resident at once).
   const int rounds = (n_items + nwg - 1) / nwg;
-  const int t_max = (P.ref_pairs + P.phases - 1) / P.phases;
-  const unsigned group_size = (unsigned)(nwg >> 3);
-  unsigned* const grp_cnt =
-      P.sync ? P.sync + (logical / (int)group_size) * kSyncStride : nullptr;
-  const bool sync_lane = (tid == (int)blockDim.x - 64);  // lane 0 of the last wave
-  unsigned arrivals = 0;
-  bool wait_ok = true;
-  auto keep_step = [&]() {
-    if (!grp_cnt) return;
-    ++arrivals;
-    if (sync_lane) {
-      __hip_atomic_fetch_add(grp_cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (wait_ok) {
-        const unsigned target = group_size * arrivals;
-        for (int spin = 0;; ++spin) {
-          if (__hip_atomic_load(grp_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
-          if (spin >= kSyncSpinMax) {
-            wait_ok = false;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-        }
-      }
-    }
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-  };
-
   for (int round = 0; round < rounds; ++round) {
     const int item = logical + round * nwg;
-    if (item >= n_items) {
-      for (int t = 0; t < t_max; ++t) keep_step();
-      continue;
-    }
+    if (item >= n_items) break;
     const int fb = item / P.phases;
     const int ph = item - fb * P.phases;
     const int n_tiles = (P.ref_pairs - ph + P.phases - 1) / P.phases;  // ref pairs ph, ph+R, ...
-    if (n_tiles <= 0) {
-      for (int t = 0; t < t_max; ++t) keep_step();
-      continue;
-    }
+    if (n_tiles <= 0) continue;
     const int64_t fit_idx = (int64_t)(FT * fb + fi) * kTile + (lane & 31);
     const bool fvalid = fit_idx < P.n_fit;
     const int list_id = (ph * 2 + ri) * 2 + (lane >> 5);
@@ -789,9 +721,7 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
       compact_lists<PER>(P.ent, my_base, cnt, thr, ovf, e2, P.k1, P.cap, fvalid);
       thr = fminf(thr, __shfl_xor(thr, 32));
       if (fvalid && P.thr && thr < thr_before) atomicMin(&P.thr[fit_idx], __float_as_uint(thr));
-      keep_step();
     }
-    for (int t = n_tiles; t < t_max; ++t) keep_step();
     if (fvalid && !DBG) {
       P.cnt[hdr] = cnt;
       P.ovf[hdr] = ovf;
@@ -1121,15 +1051,14 @@ __global__ __launch_bounds__(256) void select_rows_kernel(const double* __restri
   }
 }
 
-__global__ void init_lists_kernel(int* cnt, float* ovf, unsigned* thr, unsigned* sync,
-                                  int64_t n_hdr, int64_t n_fit) {
+__global__ void init_lists_kernel(int* cnt, float* ovf, unsigned* thr, int64_t n_hdr,
+                                  int64_t n_fit) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_hdr) {
     cnt[i] = 0;
     ovf[i] = INFINITY;
   }
   if (i < n_fit) thr[i] = kInfBits;
-  if (sync && i < kSyncGroups * kSyncStride) sync[i] = 0u;
 }
 
 __global__ void max_kernel(const double* __restrict__ v, int64_t n, double* out) {
@@ -1193,10 +1122,9 @@ hipError_t launch_prep(const DevFrames& f, const DevWeights& w, hipStream_t st) 
 
 hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st) {
   const int64_t n_hdr = n_fit * L.nl;
-  int64_t n = n_hdr > n_fit ? n_hdr : n_fit;
-  if (n < kSyncGroups * kSyncStride) n = kSyncGroups * kSyncStride;
+  const int64_t n = n_hdr > n_fit ? n_hdr : n_fit;
   hipLaunchKernelGGL(init_lists_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     L.cnt, L.ovf, L.thr, L.sync, n_hdr, n_fit);
+                     L.cnt, L.ovf, L.thr, n_hdr, n_fit);
   return hipGetLastError();
 }
 
@@ -1235,7 +1163,6 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   P.cnt = L.cnt;
   P.ovf = L.ovf;
   P.thr = L.thr;
-  P.sync = dbg_scores ? nullptr : L.sync;
   P.nl = L.nl;
   P.cap = L.cap;
   P.nofit = nofit ? 1 : 0;

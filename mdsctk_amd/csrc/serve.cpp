@@ -226,8 +226,7 @@ class Server {
       size_t ent, hdr;
       list_need(limit, q, &ent, &hdr);
       if ((rc = ensure(c_, &s.ws_lists, &s.ws_lists_bytes, ent * 8, "candidate lists"))) return rc;
-      const size_t small =
-          hdr * 8 + (size_t)limit * 12 + kSyncGroups * kSyncStride * sizeof(unsigned) + 256;
+      const size_t small = hdr * 8 + (size_t)limit * 12 + 256;
       if ((rc = ensure(c_, &s.ws_small, &s.ws_small_bytes, small, "list headers"))) return rc;
     }
     const size_t elems = (size_t)limit * q.k_out;
