@@ -279,6 +279,81 @@ def run_mixed_sets(knn, synth, oracle, args):
                             "gpu_knn_s": gpu_s, "cpu_knn_s": cpu_s}}
 
 
+COPY_PROBE_BYTES = 1 << 30
+
+
+def _pmc_child(counter, child_args, tag, workdir):
+    """One `rocprofv3 --pmc <counter>` pass over a child bench.py run; returns
+    {dispatch: (kernel name, summed counter value)} or None."""
+    import csv
+    import shutil
+    import subprocess
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    out = os.path.join(workdir, tag)
+    cmd = ["timeout", "-s", "KILL", "240", prof, "--pmc", counter, "--output-format", "csv",
+           "-d", out, "-o", "run", "--", sys.executable, os.path.abspath(__file__)] + child_args
+    env = dict(os.environ, TMPDIR="/tmp")
+    p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    path = os.path.join(out, "run_counter_collection.csv")
+    if p.returncode != 0 or not os.path.exists(path):
+        return None
+    res = {}
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        d = int(r["Dispatch_Id"])
+        name, v = res.get(d, (r["Kernel_Name"], 0.0))
+        res[d] = (name, v + float(r["Counter_Value"]))
+    return res
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of the headline contraction: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch, one rocprofv3 --pmc pass each), each scaled
+    by what the same counter reads for a plain 1 GiB device copy."""
+    import tempfile
+    work = tempfile.mkdtemp(prefix="mdsctk_pmc_", dir="/tmp")
+    probe = ["--copy-probe", str(COPY_PROBE_BYTES)]
+    child = ["--traffic-child", "--n-frames", str(args.n_frames), "--natoms", str(args.natoms),
+             "--k", str(args.k), "--precision", args.precision, "--seed", str(args.seed)]
+    r = {}
+    for tag, counter, a in (("copy_fetch", "FETCH_SIZE", probe), ("copy_write", "WRITE_SIZE", probe),
+                            ("fetch", "FETCH_SIZE", child), ("write", "WRITE_SIZE", child)):
+        log(f"traffic pass {tag} ...")
+        r[tag] = _pmc_child(counter, a, tag, work)
+        if r[tag] is None:
+            log(f"traffic pass {tag} failed; traffic not reported")
+            return None, None
+    copy_f = [v for n, v in r["copy_fetch"].values() if "copyBuffer" in n]
+    copy_w = [v for n, v in r["copy_write"].values() if "copyBuffer" in n]
+    con_f = [v for n, v in r["fetch"].values() if "contract_kernel" in n]
+    con_w = [v for n, v in r["write"].values() if "contract_kernel" in n]
+    if not (copy_f and copy_w and con_f and con_w):
+        return None, None
+    f_scale = COPY_PROBE_BYTES / (max(copy_f) * 1024.0)
+    w_scale = COPY_PROBE_BYTES / (max(copy_w) * 1024.0)
+    fetch_b = max(con_f) * 1024.0 * f_scale
+    write_b = max(con_w) * 1024.0 * w_scale
+    return fetch_b + write_b, {"fetch_bytes": fetch_b, "write_bytes": write_b,
+                               "fetch_scale": f_scale, "write_scale": w_scale,
+                               "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, KiB per dispatch, "
+                                         "scaled by a 1 GiB hipMemcpy device-to-device copy"}
+
+
+def traffic_child(args):
+    """One headline query, nothing else: the launch the PMC passes read."""
+    from mdsctk_amd import knn, synth
+    hip = Hip()
+    frames = synth.metastable_trajectory(args.n_frames, args.natoms, args.seed)
+    w = synth.masses(args.natoms, "uniform")
+    with knn.KnnContext(0, args.precision) as ctx:
+        ctx.set_reference(frames, w)
+        time_fused(ctx, hip, args.n_frames, args.k, 1, 0)
+    print(json.dumps({"traffic_child": True}), flush=True)
+
+
 def mean_stat(stats, key):
     return float(np.mean([s[key] for s in stats]))
 
@@ -303,6 +378,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=128, help="fit rows of the CPU baseline sample (0 = skip)")
     ap.add_argument("--check-rows", type=int, default=8, help="rows checked against the FP64 oracle")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--traffic", type=int, default=1,
+                    help="measure HBM traffic of the contraction with rocprofv3 PMC passes (0 = skip)")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device (HIP blit kernel) 3 times and exit: the "
                          "plain copy of known size that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -318,10 +396,20 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.copy_probe}), flush=True)
         return
 
+    if args.traffic_child:
+        traffic_child(args)
+        return
+
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the knn_rms engine runs on a single "
-              "MI355X; see DESIGN.md for how the path would shard). Use --gpus 1.", file=sys.stderr)
+              "MI355X; query rows shard across GPUs without a collective). Use --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
+
+    # PMC passes first, in child processes, before this process touches the GPU.
+    traffic, traffic_info = (None, None)
+    if args.traffic:
+        traffic, traffic_info = measure_traffic(args)
 
     from mdsctk_amd import knn, synth
     import oracle
@@ -357,9 +445,11 @@ def main():
     achieved = flops / (ms_contract * 1e-3) / 1e12
     out["roofline"] = {"bound": "mfma", "kernel": "contract_kernel",
                        "achieved": achieved, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
-                       "frac": achieved / PEAK_TFLOPS[prec], "traffic": None,
+                       "frac": achieved / PEAK_TFLOPS[prec], "traffic": traffic,
                        "flops_per_launch": flops, "ms_per_launch": ms_contract,
                        "flops_per_pair": FLOPS_PER_ATOM[prec] * natoms}
+    if traffic_info:
+        out["roofline"]["traffic_detail"] = traffic_info
     out["stages_ms"] = {s: mean_stat(stats, "ms_" + s) for s in ("prep", "contract", "merge", "fallback", "total")}
     out["candidates_per_query"] = mean_stat(stats, "n_candidates") / n
     out["fallback_rows"] = int(max(s["n_fallback_rows"] for s in stats))

@@ -585,7 +585,9 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
   __shared__ double sGr[2][2 * kTile];  // G~ of the current / next reference tile pair
 
   const int tid = threadIdx.x;
-  const int wave = tid >> 6;
+  // wave-uniform, so that everything derived from it (tile roles, copy
+  // sources, LDS slots) lives in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const int ri = wave & 1;   // reference tile of this wave
   const int fi = wave >> 1;  // fit tile of this wave
