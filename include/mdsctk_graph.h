@@ -1,7 +1,9 @@
 /*
  * mdsctk_graph.h - the stages after knn_rms: the symmetric sparse kNN
  * graph (make_sysparse) and its leading spectral decomposition
- * (decomp_sparse).
+ * (decomp_sparse); the general sparse matrix of out-of-sample rows
+ * (make_gesparse) and the Nystrom extension of the eigenvectors to them
+ * (decomp_sparse_nystrom).
  *
  * .ssm layout (make_sysparse.cpp:285-316, read back by CSC_matrix at
  * mdsctk.cpp:44-59 and decomp_sparse.cpp:137-145), host byte order,
@@ -55,6 +57,36 @@ int mdsctk_ssm_read(const char* path, int32_t* n, int64_t* nnz, int32_t* pcol, i
 int mdsctk_spectral_decomp(int32_t n, const int32_t* pcol, const int32_t* irow,
                            const double* dist, double sigma, int32_t nev, double* evals,
                            double* evecs, double* residuals);
+
+/* General sparse matrix of out-of-sample kNN rows, make_gesparse semantics
+ * (make_gesparse.cpp:226-333, split_edges at mdsctk.cpp:580-603).  The .gsm
+ * file has the .ssm layout (mdsctk_ssm_write / mdsctk_ssm_read handle it)
+ * with one column per query row: column i holds the entries (i, j) for the
+ * first k of maxk neighbours j of row i, rows ascending, values the kNN
+ * distances; j indexes the reference set and may exceed n_rows; i == j is
+ * kept.  A row's own entry replaces an earlier one of the same edge.
+ * With symmetric != 0 (make_gesparse -s) the mirrored entry (j, i) is added
+ * when that edge has no entry yet; mirrored entries with j >= n_rows have no
+ * column and are dropped, as in the reference's conversion loop.
+ * Call with pcol/irow/vals NULL to get *nnz; then with buffers of n_rows+1
+ * and *nnz entries. */
+int mdsctk_gsm_build(const double* dist, const int32_t* idx, int64_t n_rows, int32_t maxk,
+                     int32_t k, int symmetric, int32_t* pcol, int32_t* irow, double* vals,
+                     int64_t* nnz);
+
+/* decomp_sparse_nystrom (decomp_sparse_nystrom.cpp:147-304): the leading
+ * eigenpairs of the in-sample .ssm matrix (n x n) as mdsctk_spectral_decomp,
+ * plus their Nystrom extension to the m out-of-sample columns of a .gsm
+ * matrix whose row indices lie in [0, n):
+ *   b_ij = exp(-d_ij^2 / 2 sigma^2) * d_i^-1/2 * (sum_i' b_i'j)^-1/2
+ *   ext[x][j] = sum_i b_ij z_x[i] / l_x
+ * with d_i the in-sample degrees.  evecs [nev][n] may be NULL; ext is
+ * [nev][m].  Returns the number of converged eigenpairs (== nev on
+ * success). */
+int mdsctk_nystrom_decomp(int32_t n, const int32_t* pcol_a, const int32_t* irow_a,
+                          const double* dist_a, int32_t m, const int32_t* pcol_b,
+                          const int32_t* irow_b, const double* dist_b, double sigma, int32_t nev,
+                          double* evals, double* evecs, double* ext, double* residuals);
 
 #ifdef __cplusplus
 }

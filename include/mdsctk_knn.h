@@ -197,8 +197,8 @@ int mdsctk_knn_wait(mdsctk_knn_ctx* ctx, int64_t request_id, mdsctk_knn_request_
 /* Counters of the serving loop since the first submit. */
 int mdsctk_knn_serve_stats_get(mdsctk_knn_ctx* ctx, mdsctk_knn_serve_stats* out);
 
-/* The host clock the request times use: milliseconds since the context was
- * created (steady clock). */
+/* The host clock the request times use: milliseconds of the steady clock
+ * since the library was loaded, the same for every context. */
 double mdsctk_knn_clock_ms(const mdsctk_knn_ctx* ctx);
 
 /* Statistics of the last completed query. */

@@ -25,6 +25,8 @@ LIB_PATH = os.path.join(LIB_DIR, "libmdsctk_knn.so")
 CLI_PATH = os.path.join(BIN_DIR, "knn_rms")
 MAKE_SYSPARSE_PATH = os.path.join(BIN_DIR, "make_sysparse")
 DECOMP_SPARSE_PATH = os.path.join(BIN_DIR, "decomp_sparse")
+MAKE_GESPARSE_PATH = os.path.join(BIN_DIR, "make_gesparse")
+DECOMP_SPARSE_NYSTROM_PATH = os.path.join(BIN_DIR, "decomp_sparse_nystrom")
 
 ARCH = os.environ.get("MDSCTK_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -37,7 +39,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
 LIB_SOURCES = ["knn_kernels.hip", "api.cpp", "serve.cpp", "xtc.cpp", "topology.cpp", "spectral.cpp"]
 # executables: (name, main source)
 TOOLS = [("knn_rms", "knn_rms_main.cpp"), ("make_sysparse", "make_sysparse_main.cpp"),
-         ("decomp_sparse", "decomp_sparse_main.cpp")]
+         ("decomp_sparse", "decomp_sparse_main.cpp"), ("make_gesparse", "make_gesparse_main.cpp"),
+         ("decomp_sparse_nystrom", "decomp_sparse_nystrom_main.cpp")]
 
 
 def _run(cmd, verbose):

@@ -257,8 +257,15 @@ void slot_release(mdsctk_knn_ctx* c, Slot& s) {
   s = Slot();
 }
 
-double clock_ms(const mdsctk_knn_ctx* c) {
-  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c->t0)
+namespace {
+// One epoch for the whole process, so that request times of different
+// contexts compare directly.
+const std::chrono::steady_clock::time_point g_clock_epoch = std::chrono::steady_clock::now();
+}  // namespace
+
+double clock_ms(const mdsctk_knn_ctx*) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                   g_clock_epoch)
       .count();
 }
 

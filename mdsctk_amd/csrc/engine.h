@@ -82,7 +82,6 @@ struct mdsctk_knn_ctx {
   mdsctk_knn_stats stats = {};
   mdsctk::Server* server = nullptr;
   std::mutex server_mu;  // guards creating / stopping the serving thread
-  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 };
 
 namespace mdsctk {

@@ -241,30 +241,51 @@ int mdsctk_ssm_read(const char* path, int32_t* n_out, int64_t* nnz_out, int32_t*
   return 0;
 }
 
-int mdsctk_spectral_decomp(int32_t n, const int32_t* pcol, const int32_t* irow,
-                           const double* dist, double sigma, int32_t nev, double* evals,
-                           double* evecs, double* residuals) {
-  if (n <= 0 || !pcol || !evals || nev <= 0 || nev > n || !(sigma > 0))
-    return io_fail(MDSCTK_ERR_ARG, "spectral_decomp: need n > 0, 0 < nev <= n, sigma > 0");
-  const int64_t nnz = pcol[n];
-  if (nnz > 0 && (!irow || !dist)) return io_fail(MDSCTK_ERR_ARG, "spectral_decomp: null arrays");
-  for (int32_t c = 0; c < n; ++c)
-    for (int32_t e = pcol[c]; e < pcol[c + 1]; ++e)
-      if (irow[e] < 0 || irow[e] >= n)
-        return io_fail(MDSCTK_ERR_FORMAT, "spectral_decomp: row index %d out of range", irow[e]);
+}  // extern "C"
 
-  // Affinities and their symmetric normalisation (decomp_sparse.cpp:177-205).
-  std::vector<double> a((size_t)nnz), deg(n, 0.0);
+namespace {
+
+// Checks a CSC matrix of `cols` columns whose row indices must lie in
+// [0, rows); returns 0 or an error code (text set).
+int check_csc(const char* who, int32_t cols, int32_t rows, const int32_t* pcol,
+              const int32_t* irow, const double* dist) {
+  if (cols <= 0 || !pcol) return io_fail(MDSCTK_ERR_ARG, "%s: empty matrix", who);
+  if (pcol[0] != 0) return io_fail(MDSCTK_ERR_FORMAT, "%s: pcol[0] must be 0", who);
+  for (int32_t c = 0; c < cols; ++c)
+    if (pcol[c + 1] < pcol[c]) return io_fail(MDSCTK_ERR_FORMAT, "%s: pcol decreases", who);
+  if (pcol[cols] > 0 && (!irow || !dist)) return io_fail(MDSCTK_ERR_ARG, "%s: null arrays", who);
+  for (int32_t c = 0; c < cols; ++c)
+    for (int32_t e = pcol[c]; e < pcol[c + 1]; ++e)
+      if (irow[e] < 0 || irow[e] >= rows)
+        return io_fail(MDSCTK_ERR_FORMAT, "%s: row index %d out of range", who, irow[e]);
+  return 0;
+}
+
+// Gaussian affinities a = exp(-d^2 / 2 sigma^2) of the stored (strict lower
+// triangle) edges, normalised as D^-1/2 A D^-1/2 with the degrees of the
+// full symmetric matrix (decomp_sparse.cpp:177-205).  dinv[i] = d_i^-1/2.
+void normalised_affinity(int32_t n, const int32_t* pcol, const int32_t* irow, const double* dist,
+                         double sigma, std::vector<double>& a, std::vector<double>& dinv) {
+  const int64_t nnz = pcol[n];
+  a.resize((size_t)nnz);
+  dinv.assign(n, 0.0);
   for (int64_t e = 0; e < nnz; ++e) a[e] = std::exp(-(dist[e] * dist[e]) / (2.0 * sigma * sigma));
   for (int32_t c = 0; c < n; ++c)
     for (int32_t e = pcol[c]; e < pcol[c + 1]; ++e) {
-      deg[c] += a[e];
-      deg[irow[e]] += a[e];
+      dinv[c] += a[e];
+      dinv[irow[e]] += a[e];
     }
-  for (int32_t i = 0; i < n; ++i) deg[i] = 1.0 / std::sqrt(deg[i]);
+  for (int32_t i = 0; i < n; ++i) dinv[i] = 1.0 / std::sqrt(dinv[i]);
   for (int32_t c = 0; c < n; ++c)
-    for (int32_t e = pcol[c]; e < pcol[c + 1]; ++e) a[e] *= deg[irow[e]] * deg[c];
+    for (int32_t e = pcol[c]; e < pcol[c + 1]; ++e) a[e] *= dinv[irow[e]] * dinv[c];
+}
 
+// Leading nev eigenpairs (largest algebraic first) of the symmetric matrix
+// held as a strict lower triangle in (pcol, irow, a).  Returns the number of
+// converged pairs.
+int leading_eigenpairs(int32_t n, const int32_t* pcol, const int32_t* irow,
+                       const std::vector<double>& a, int32_t nev, double* evals, double* evecs,
+                       double* residuals) {
   // Thick-restart Lanczos.
   const int m = (int)std::min<int64_t>((int64_t)10 * nev + 1, n);
   const double tol = 1e-13;
@@ -383,6 +404,122 @@ int mdsctk_spectral_decomp(int32_t n, const int32_t* pcol, const int32_t* irow,
     }
   }
   return converged >= nev ? nev : converged;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdsctk_spectral_decomp(int32_t n, const int32_t* pcol, const int32_t* irow,
+                           const double* dist, double sigma, int32_t nev, double* evals,
+                           double* evecs, double* residuals) {
+  if (n <= 0 || !pcol || !evals || nev <= 0 || nev > n || !(sigma > 0))
+    return io_fail(MDSCTK_ERR_ARG, "spectral_decomp: need n > 0, 0 < nev <= n, sigma > 0");
+  if (int rc = check_csc("spectral_decomp", n, n, pcol, irow, dist)) return rc;
+  std::vector<double> a, dinv;
+  normalised_affinity(n, pcol, irow, dist, sigma, a, dinv);
+  return leading_eigenpairs(n, pcol, irow, a, nev, evals, evecs, residuals);
+}
+
+int mdsctk_gsm_build(const double* dist, const int32_t* idx, int64_t n_rows, int32_t maxk,
+                     int32_t k, int symmetric, int32_t* pcol, int32_t* irow, double* vals,
+                     int64_t* nnz) {
+  if (!dist || !idx || !nnz || n_rows < 0 || n_rows > INT32_MAX || maxk <= 0 || k <= 0)
+    return io_fail(MDSCTK_ERR_ARG, "gsm_build: need rows, maxk > 0 and k > 0");
+  if (k > maxk)
+    return io_fail(MDSCTK_ERR_ARG, "gsm_build: output k (%d) is larger than the input k (%d)", k,
+                   maxk);
+  // kind 0: the row's own entry (a later one replaces an earlier one);
+  // kind 1: the mirrored entry of -s, which only fills an edge that has no
+  // entry yet, so the row's own entry always wins and the first mirror does.
+  struct GEdge {
+    int32_t from, to;
+    int32_t kind;
+    int64_t order;
+    double d;
+  };
+  std::vector<GEdge> edges;
+  edges.reserve((size_t)n_rows * k * (symmetric ? 2 : 1));
+  int64_t order = 0;
+  for (int64_t i = 0; i < n_rows; ++i)
+    for (int32_t x = 0; x < k; ++x) {
+      const int32_t j = idx[i * maxk + x];
+      const double d = dist[i * maxk + x];
+      if (j < 0)
+        return io_fail(MDSCTK_ERR_ARG, "gsm_build: row %lld holds a negative index %d",
+                       (long long)i, j);
+      edges.push_back(GEdge{(int32_t)i, j, 0, order++, d});
+      // columns are query rows: a mirrored edge from a reference index
+      // beyond the last row has no column to live in
+      if (symmetric && j < n_rows) edges.push_back(GEdge{j, (int32_t)i, 1, order++, d});
+    }
+  std::sort(edges.begin(), edges.end(), [](const GEdge& a, const GEdge& b) {
+    if (a.from != b.from) return a.from < b.from;
+    if (a.to != b.to) return a.to < b.to;
+    if (a.kind != b.kind) return a.kind < b.kind;
+    return a.kind == 0 ? a.order > b.order : a.order < b.order;
+  });
+  size_t m = 0;
+  for (size_t e = 0; e < edges.size(); ++e)
+    if (m == 0 || edges[m - 1].from != edges[e].from || edges[m - 1].to != edges[e].to)
+      edges[m++] = edges[e];
+  edges.resize(m);
+  *nnz = (int64_t)m;
+  if (!pcol && !irow && !vals) return 0;
+  if (!pcol || !irow || !vals) return io_fail(MDSCTK_ERR_ARG, "gsm_build: null output buffer");
+  if (m > (size_t)INT32_MAX) return io_fail(MDSCTK_ERR_ARG, "gsm_build: too many edges");
+  std::fill(pcol, pcol + n_rows + 1, 0);
+  for (const GEdge& e : edges) ++pcol[e.from + 1];
+  for (int64_t c = 0; c < n_rows; ++c) pcol[c + 1] += pcol[c];
+  for (size_t e = 0; e < m; ++e) {
+    irow[e] = edges[e].to;
+    vals[e] = edges[e].d;
+  }
+  return 0;
+}
+
+int mdsctk_nystrom_decomp(int32_t n, const int32_t* pcol_a, const int32_t* irow_a,
+                          const double* dist_a, int32_t m, const int32_t* pcol_b,
+                          const int32_t* irow_b, const double* dist_b, double sigma, int32_t nev,
+                          double* evals, double* evecs, double* ext, double* residuals) {
+  if (n <= 0 || m <= 0 || !evals || !ext || nev <= 0 || nev > n || !(sigma > 0))
+    return io_fail(MDSCTK_ERR_ARG,
+                   "nystrom_decomp: need n > 0, m > 0, 0 < nev <= n, sigma > 0, ext buffer");
+  if (int rc = check_csc("nystrom_decomp (ssm)", n, n, pcol_a, irow_a, dist_a)) return rc;
+  if (int rc = check_csc("nystrom_decomp (gsm)", m, n, pcol_b, irow_b, dist_b)) return rc;
+  std::vector<double> a, dinv;
+  normalised_affinity(n, pcol_a, irow_a, dist_a, sigma, a, dinv);
+  std::vector<double> z_own;
+  if (!evecs) {
+    z_own.resize((size_t)nev * n);
+    evecs = z_own.data();
+  }
+  const int got = leading_eigenpairs(n, pcol_a, irow_a, a, nev, evals, evecs, residuals);
+  // Out-of-sample affinities, normalised by the in-sample degrees of their
+  // reference rows and their own column sums (decomp_sparse_nystrom.cpp:
+  // 182-215), then ext = B^T (z / l) column by column (:297-304).
+  const int64_t nnz_b = pcol_b[m];
+  std::vector<double> b((size_t)nnz_b);
+  for (int64_t e = 0; e < nnz_b; ++e)
+    b[e] = std::exp(-(dist_b[e] * dist_b[e]) / (2.0 * sigma * sigma));
+  for (int32_t c = 0; c < m; ++c) {
+    double sum = 0.0;
+    for (int32_t e = pcol_b[c]; e < pcol_b[c + 1]; ++e) sum += b[e];
+    const double dc = 1.0 / std::sqrt(sum);
+    for (int32_t e = pcol_b[c]; e < pcol_b[c + 1]; ++e) b[e] *= dinv[irow_b[e]] * dc;
+  }
+  std::vector<double> zl(n);
+  for (int x = 0; x < nev && x < got; ++x) {
+    const double* z = evecs + (size_t)x * n;
+    for (int32_t r = 0; r < n; ++r) zl[r] = z[r] / evals[x];
+    double* y = ext + (size_t)x * m;
+    for (int32_t c = 0; c < m; ++c) {
+      double acc = 0.0;
+      for (int32_t e = pcol_b[c]; e < pcol_b[c + 1]; ++e) acc += zl[irow_b[e]] * b[e];
+      y[c] = acc;
+    }
+  }
+  return got;
 }
 
 }  // extern "C"

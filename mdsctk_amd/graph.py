@@ -1,5 +1,7 @@
 """Python binding of include/mdsctk_graph.h: the .ssm kNN graph
-(make_sysparse) and its leading spectral decomposition (decomp_sparse)."""
+(make_sysparse) and its leading spectral decomposition (decomp_sparse); the
+.gsm matrix of out-of-sample rows (make_gesparse) and the Nystrom extension
+(decomp_sparse_nystrom)."""
 from __future__ import annotations
 
 import ctypes
@@ -25,6 +27,12 @@ def _lib():
         L.mdsctk_ssm_read.restype = ctypes.c_int
         L.mdsctk_spectral_decomp.argtypes = [i32, vp, vp, vp, ctypes.c_double, i32, vp, vp, vp]
         L.mdsctk_spectral_decomp.restype = ctypes.c_int
+        L.mdsctk_gsm_build.argtypes = [vp, vp, i64, i32, i32, ctypes.c_int, vp, vp, vp,
+                                       ctypes.POINTER(i64)]
+        L.mdsctk_gsm_build.restype = ctypes.c_int
+        L.mdsctk_nystrom_decomp.argtypes = [i32, vp, vp, vp, i32, vp, vp, vp, ctypes.c_double, i32,
+                                            vp, vp, vp, vp]
+        L.mdsctk_nystrom_decomp.restype = ctypes.c_int
         L.mdsctk_io_last_error.restype = ctypes.c_char_p
         _bound = True
     return L
@@ -41,7 +49,9 @@ def _check(L, rc):
 
 
 class SSM:
-    """Strict lower triangle of a symmetric matrix in CSC form."""
+    """A CSC matrix as the .ssm / .gsm files hold it: n columns, row indices
+    irow, values vals.  For an .ssm it is the strict lower triangle of a
+    symmetric matrix; for a .gsm one column per out-of-sample row."""
 
     def __init__(self, n, pcol, irow, vals):
         self.n, self.pcol, self.irow, self.vals = int(n), pcol, irow, vals
@@ -97,3 +107,42 @@ def spectral_decomp(m: SSM, sigma, nev, vectors=True):
     got = _check(L, L.mdsctk_spectral_decomp(m.n, _p(m.pcol), _p(m.irow), _p(m.vals), float(sigma),
                                              nev, _p(ev), _p(vec), _p(res)))
     return ev, vec, res, got
+
+
+def gsm_build(dist, idx, k=None, symmetric=False) -> SSM:
+    """make_gesparse on kNN rows dist[m, maxk] (float64), idx[m, maxk] (int32):
+    one column per row (``-s``: mirrored entries fill absent edges)."""
+    L = _lib()
+    dist = np.ascontiguousarray(dist, np.float64)
+    idx = np.ascontiguousarray(idx, np.int32)
+    n, maxk = idx.shape
+    k = maxk if k is None else k
+    sym = 1 if symmetric else 0
+    nnz = ctypes.c_int64()
+    _check(L, L.mdsctk_gsm_build(_p(dist), _p(idx), n, maxk, k, sym, None, None, None,
+                                 ctypes.byref(nnz)))
+    pcol = np.zeros(n + 1, np.int32)
+    irow = np.zeros(nnz.value, np.int32)
+    vals = np.zeros(nnz.value, np.float64)
+    _check(L, L.mdsctk_gsm_build(_p(dist), _p(idx), n, maxk, k, sym, _p(pcol), _p(irow), _p(vals),
+                                 ctypes.byref(nnz)))
+    return SSM(n, pcol, irow, vals)
+
+
+gsm_write = ssm_write
+gsm_read = ssm_read
+
+
+def nystrom_decomp(a: SSM, b: SSM, sigma, nev, vectors=True):
+    """Leading nev eigenpairs of the in-sample graph ``a`` and their Nystrom
+    extension to the out-of-sample columns of ``b``.  Returns (evals
+    descending, evecs[nev, n] or None, ext[nev, m], residuals, converged)."""
+    L = _lib()
+    ev = np.zeros(nev, np.float64)
+    vec = np.zeros((nev, a.n), np.float64) if vectors else None
+    ext = np.zeros((nev, b.n), np.float64)
+    res = np.zeros(nev, np.float64)
+    got = _check(L, L.mdsctk_nystrom_decomp(a.n, _p(a.pcol), _p(a.irow), _p(a.vals), b.n,
+                                            _p(b.pcol), _p(b.irow), _p(b.vals), float(sigma), nev,
+                                            _p(ev), _p(vec), _p(ext), _p(res)))
+    return ev, vec, ext, res, got
