@@ -207,6 +207,40 @@ int mdsctk_knn_get_stats(const mdsctk_knn_ctx* ctx, mdsctk_knn_stats* out);
 /* Number of entries per row a query with this k and flags returns. */
 int mdsctk_knn_row_width(const mdsctk_knn_ctx* ctx, int32_t k, uint32_t flags);
 
+/* ---- knn_data: data vectors instead of frames.
+ *
+ * Replaces knn_data's pair loop (knn_data.cpp:196-237) with its distance
+ * functions euclidean_distance / correlation_distance (mdsctk.cpp:329-356,
+ * selected by -c at knn_data.cpp:103-105) on the same engine: contraction
+ * for candidates, FP64 re-score of the certified window, (distance, index)
+ * order, k clamp, drop-rank-0 and --sort false layout as mdsctk_knn_query.
+ * A context holds either frames or data vectors; the frame entry points
+ * (and serving) refuse a context holding data vectors and the reverse.
+ * Precision forms: split-BF16, FP32 or exact (FP16 is refused: arbitrary
+ * data ranges do not fit it). */
+#define MDSCTK_METRIC_EUCLIDEAN 1   /* sqrt(sum (a-b)^2) */
+#define MDSCTK_METRIC_CORRELATION 2 /* sqrt(max(0, (1 - r) / 2)), r the Pearson correlation */
+
+/* Resident set of n_ref vectors x[n_ref][vdim] (FP64, as the .pts file
+ * holds them).  Non-finite values, and constant vectors under the
+ * correlation metric, are rejected with MDSCTK_ERR_ARG. */
+int mdsctk_data_set_reference(mdsctk_knn_ctx* ctx, const double* x, int64_t n_ref, int32_t vdim,
+                              int metric);
+
+/* k nearest of each query vector x[n_fit][vdim] (NULL = self query) in the
+ * resident set; flags MDSCTK_KNN_FULL_ROWS / MDSCTK_KNN_KEEP_RANK0 as for
+ * mdsctk_knn_query (MDSCTK_KNN_NOFIT is refused).  Distances are in the data's
+ * units.  Returns the number of entries written per row. */
+int mdsctk_data_query(mdsctk_knn_ctx* ctx, const double* x, int64_t n_fit, int32_t k,
+                      uint32_t flags, double* out_dist, int32_t* out_idx);
+
+/* Diagnostic (no counterpart in the reference): approximate and exact
+ * scores [n_fit][n_ref] of data vectors, as mdsctk_knn_debug_scores
+ * (s = squared Euclidean distance, or 2 (1 - r) for the correlation
+ * metric). */
+int mdsctk_data_debug_scores(mdsctk_knn_ctx* ctx, const double* x, int64_t n_fit,
+                             double* approx_s, double* exact_s);
+
 #ifdef __cplusplus
 }
 #endif

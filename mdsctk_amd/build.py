@@ -26,6 +26,7 @@ CLI_PATH = os.path.join(BIN_DIR, "knn_rms")
 MAKE_SYSPARSE_PATH = os.path.join(BIN_DIR, "make_sysparse")
 DECOMP_SPARSE_PATH = os.path.join(BIN_DIR, "decomp_sparse")
 MAKE_GESPARSE_PATH = os.path.join(BIN_DIR, "make_gesparse")
+KNN_DATA_PATH = os.path.join(BIN_DIR, "knn_data")
 DECOMP_SPARSE_NYSTROM_PATH = os.path.join(BIN_DIR, "decomp_sparse_nystrom")
 
 ARCH = os.environ.get("MDSCTK_OFFLOAD_ARCH", "gfx950")
@@ -40,7 +41,8 @@ LIB_SOURCES = ["knn_kernels.hip", "api.cpp", "serve.cpp", "xtc.cpp", "topology.c
 # executables: (name, main source)
 TOOLS = [("knn_rms", "knn_rms_main.cpp"), ("make_sysparse", "make_sysparse_main.cpp"),
          ("decomp_sparse", "decomp_sparse_main.cpp"), ("make_gesparse", "make_gesparse_main.cpp"),
-         ("decomp_sparse_nystrom", "decomp_sparse_nystrom_main.cpp")]
+         ("decomp_sparse_nystrom", "decomp_sparse_nystrom_main.cpp"),
+         ("knn_data", "knn_data_main.cpp")]
 
 
 def _run(cmd, verbose):

@@ -34,6 +34,24 @@ double kappa_for(int prec, int natoms) {
   }
 }
 
+// Window of data vectors.  Their value distributions are arbitrary (few
+// elements, no averaging over atoms), so the split forms' representation
+// term is the proven worst case instead of the fitted constant: with
+// A = hi + lo and r = x - A (|lo| <= 2^-8 |x|, |r| <= 2^-16 |x|),
+// s~ = sum (A_a - A_b)^2 + 2 sum lo_a lo_b, and Cauchy-Schwarz bounds
+// |s~ - |a - b|^2| by 5 * 2^-16 (G_a + G_b).  Rounding the FP64 vectors to
+// the FP32 source values adds at most 2^-22 (G_a + G_b) (|delta| <= 2^-24
+// |x|, s <= 2 (G_a + G_b)).  FP32 accumulation keeps the fitted sqrt(N) term.
+double kappa_data(int prec, int natoms) {
+  const double rn = std::sqrt((double)std::max(natoms, 16));
+  const double input = 2.4e-7;
+  switch (prec) {
+    case kFP32: return 2.0 * (5.5e-8 * rn) + input;
+    case kBF16: return 2.0 * (3.0e-8 * rn) + 5.0 * 1.5259e-5 + input;
+    default: return 0.0;
+  }
+}
+
 size_t operand_bytes(int prec, int n_tiles, int n_groups) {
   return ((size_t)n_tiles * n_groups + kOperandTailGroups) * blocks_per_group(prec) * 64 * 16;
 }
@@ -130,6 +148,8 @@ namespace {
 
 void free_frames(mdsctk_knn_ctx* c, DevFrames& f) {
   if (f.owns_xyz) dfree(c, f.xyz);
+  dfree(c, f.x64);
+  dfree(c, f.aux);
   dfree(c, f.cen);
   dfree(c, f.G);
   dfree(c, f.Gt);
@@ -215,6 +235,8 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
   server_stop(c);
   HIP_TRY(c, hipSetDevice(c->device));
   free_frames(c, c->ref);
+  dfree(c, c->data_shift);
+  c->data_shift = nullptr;
   c->has_ref = false;
   int rc = set_weights(c, weights, natoms);
   if (rc) return rc;
@@ -227,6 +249,84 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
   if ((rc = alloc_frames(c, c->ref, n_ref, natoms, c->prec))) return rc;
   if ((rc = prep_frames(c, c->ref, c->main.st))) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->main.st));
+  c->has_ref = true;
+  return 0;
+}
+
+// Host-side checks of data vectors: finite values, and for the correlation
+// metric no constant vector (its standard deviation is zero and
+// correlation_distance divides by it).
+int check_data(mdsctk_knn_ctx* c, const double* x, int64_t n, int vdim, int metric,
+               const char* who) {
+  for (int64_t i = 0; i < n; ++i) {
+    const double* r = x + i * (int64_t)vdim;
+    double lo = r[0], hi = r[0];
+    for (int e = 0; e < vdim; ++e) {
+      if (!std::isfinite(r[e]))
+        return fail(c, MDSCTK_ERR_ARG, "%s: vector %lld holds a non-finite value", who,
+                    (long long)i);
+      lo = std::min(lo, r[e]);
+      hi = std::max(hi, r[e]);
+    }
+    if (metric == kMetricCorrelation && !(hi > lo))
+      return fail(c, MDSCTK_ERR_ARG,
+                  "%s: vector %lld is constant, so its correlation distance is undefined", who,
+                  (long long)i);
+  }
+  return 0;
+}
+
+int set_data_reference_impl(mdsctk_knn_ctx* c, const double* x, int64_t n_ref, int32_t vdim,
+                            int metric) {
+  if (!c) return MDSCTK_ERR_ARG;
+  if (metric != kMetricEuclidean && metric != kMetricCorrelation)
+    return fail(c, MDSCTK_ERR_ARG, "set_data_reference: unknown metric %d", metric);
+  if (!x || n_ref <= 0 || vdim <= 0 || (metric == kMetricCorrelation && vdim < 2))
+    return fail(c, MDSCTK_ERR_ARG, "set_data_reference: need vectors (n_ref=%lld vdim=%d)",
+                (long long)n_ref, vdim);
+  if (n_ref > INT32_MAX) return fail(c, MDSCTK_ERR_ARG, "n_ref exceeds the int32 index range");
+  if (c->prec == kFP16)
+    return fail(c, MDSCTK_ERR_ARG,
+                "set_data_reference: data vectors take the split-BF16, FP32 or exact form "
+                "(FP16 cannot hold arbitrary value ranges)");
+  int rc = check_data(c, x, n_ref, vdim, metric, "set_data_reference");
+  if (rc) return rc;
+  if (!server_idle(c))
+    return fail(c, MDSCTK_ERR_STATE, "set_data_reference: served requests are still outstanding");
+  server_stop(c);
+  HIP_TRY(c, hipSetDevice(c->device));
+  free_frames(c, c->ref);
+  dfree(c, c->data_shift);
+  c->data_shift = nullptr;
+  c->has_ref = false;
+  const int natoms = (vdim + 2) / 3;
+  if ((rc = set_weights(c, nullptr, natoms))) return rc;
+  c->w.metric = metric;
+  c->w.out_scale = 1.0;
+  c->w.W = metric == kMetricCorrelation ? 4.0 : 1.0;
+  DevFrames& f = c->ref;
+  hipStream_t st = c->main.st;
+  if ((rc = dalloc(c, (void**)&f.x64, (size_t)n_ref * vdim * sizeof(double), "reference vectors")))
+    return rc;
+  HIP_TRY(c, hipMemcpyAsync(f.x64, x, (size_t)n_ref * vdim * sizeof(double),
+                            hipMemcpyHostToDevice, st));
+  if (metric == kMetricCorrelation &&
+      (rc = dalloc(c, (void**)&f.aux, (size_t)n_ref * 2 * sizeof(double), "vector statistics")))
+    return rc;
+  if ((rc = dalloc(c, (void**)&f.xyz, (size_t)n_ref * natoms * 3 * sizeof(float),
+                   "reference operand source")))
+    return rc;
+  f.owns_xyz = true;
+  if ((rc = alloc_frames(c, f, n_ref, natoms, c->prec))) return rc;
+  f.vdim = vdim;
+  if (metric == kMetricEuclidean) {
+    if ((rc = dalloc(c, (void**)&c->data_shift, (size_t)vdim * sizeof(double), "mean vector")))
+      return rc;
+    HIP_TRY(c, launch_column_mean(f.x64, n_ref, vdim, c->data_shift, st));
+  }
+  HIP_TRY(c, launch_data_prep(f, metric, c->data_shift, st));
+  if ((rc = prep_frames(c, f, st))) return rc;
+  HIP_TRY(c, hipStreamSynchronize(st));
   c->has_ref = true;
   return 0;
 }
@@ -271,7 +371,8 @@ double clock_ms(const mdsctk_knn_ctx*) {
 
 int resolve_shape(mdsctk_knn_ctx* c, int32_t k, uint32_t flags, QueryShape* q) {
   const int64_t n_ref = c->ref.n;
-  q->nofit = flags & MDSCTK_KNN_NOFIT;
+  // data vectors take the trace of the 3x3 block (their dot product)
+  q->nofit = (flags & MDSCTK_KNN_NOFIT) || c->w.metric != kMetricRMSD;
   q->full = flags & MDSCTK_KNN_FULL_ROWS;
   q->rank_off = (flags & MDSCTK_KNN_KEEP_RANK0) ? 0 : 1;
   if (!q->full && k < 0) return fail(c, MDSCTK_ERR_ARG, "query: k must be >= 0");
@@ -289,8 +390,10 @@ int slot_frames(mdsctk_knn_ctx* c, Slot& s, int64_t n_fit) {
   DevFrames& f = s.fit;
   const int natoms = c->ref.natoms;
   const int want_prec = c->prec == kExact ? kFP32 : c->prec;
+  const int metric = c->w.metric;
+  const int vdim = metric == kMetricRMSD ? 0 : c->ref.vdim;
   if (f.cap_tiles == 0 || (size_t)n_fit > s.fit_cap_frames || s.fit_natoms != natoms ||
-      f.prec != want_prec) {
+      f.prec != want_prec || s.fit_vdim != vdim || s.fit_metric != metric) {
     free_frames(c, f);
     s.fit_cap_frames = 0;
     int rc;
@@ -299,8 +402,18 @@ int slot_frames(mdsctk_knn_ctx* c, Slot& s, int64_t n_fit) {
       return rc;
     f.owns_xyz = true;
     if ((rc = alloc_frames(c, f, std::max<int64_t>(n_fit, 1), natoms, c->prec))) return rc;
-    s.fit_cap_frames = (size_t)std::max<int64_t>(n_fit, 1);
+    const size_t cap = (size_t)std::max<int64_t>(n_fit, 1);
+    if (vdim) {
+      if ((rc = dalloc(c, (void**)&f.x64, cap * vdim * sizeof(double), "query vectors"))) return rc;
+      if (metric == kMetricCorrelation &&
+          (rc = dalloc(c, (void**)&f.aux, cap * 2 * sizeof(double), "vector statistics")))
+        return rc;
+    }
+    f.vdim = vdim;
+    s.fit_cap_frames = cap;
     s.fit_natoms = natoms;
+    s.fit_vdim = vdim;
+    s.fit_metric = metric;
   }
   // Reuse the allocation: a smaller batch only changes the row and tile
   // counts (the kernels walk n_tiles; the slack beyond it is never read).
@@ -310,6 +423,21 @@ int slot_frames(mdsctk_knn_ctx* c, Slot& s, int64_t n_fit) {
 }
 
 int prep_slot(mdsctk_knn_ctx* c, Slot& s) { return prep_frames(c, s.fit, s.st); }
+
+int stage_input(mdsctk_knn_ctx* c, Slot& s, const void* in, bool on_device, int64_t n_fit) {
+  int rc = slot_frames(c, s, n_fit);
+  if (rc) return rc;
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (c->w.metric == kMetricRMSD) {
+    HIP_TRY(c, hipMemcpyAsync(s.fit.xyz, in, (size_t)n_fit * c->ref.natoms * 3 * sizeof(float),
+                              kind, s.st));
+  } else {
+    HIP_TRY(c, hipMemcpyAsync(s.fit.x64, in, (size_t)n_fit * c->ref.vdim * sizeof(double), kind,
+                              s.st));
+    HIP_TRY(c, launch_data_prep(s.fit, c->w.metric, c->data_shift, s.st));
+  }
+  return prep_slot(c, s);
+}
 
 int contract_grid(const mdsctk_knn_ctx* c) {
   // workgroups resident at once: two 4-wave or one 8-wave per CU
@@ -363,7 +491,9 @@ int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist
   L.thr = (unsigned*)sp; sp += n_fit * sizeof(unsigned);
   float* eps2 = (float*)sp; sp += n_fit * sizeof(float);
   int* d_fb = (int*)sp;
-  const double kappa = kappa_for(c->prec, c->ref.natoms) * c->eps_scale;
+  const double kappa = (c->w.metric == kMetricRMSD ? kappa_for(c->prec, c->ref.natoms)
+                                                   : kappa_data(c->prec, c->ref.natoms)) *
+                       c->eps_scale;
   HIP_TRY(c, launch_init_lists(L, n_fit, s.st));
   HIP_TRY(c, launch_eps(fit, c->ref, kappa, eps2, s.gmax, s.st));
   HIP_TRY(c, hipMemsetAsync(s.n_cand, 0, sizeof(int), s.st));
@@ -398,7 +528,7 @@ int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const in
     for (int i = 0; i < nb; ++i) ids[i] = h_rows ? h_rows[b0 + i] : b0 + i;
     HIP_TRY(c, hipMemcpyAsync(drows, ids.data(), nb * sizeof(int), hipMemcpyHostToDevice, s.st));
     HIP_TRY(c, launch_exact_rows(c->ref, fit, c->w, drows, nb, q.nofit, srow, nullptr, 0, s.st));
-    HIP_TRY(c, launch_select_rows(srow, n_ref, drows, nb, c->w.W, q.k1, q.k_out, q.rank_off,
+    HIP_TRY(c, launch_select_rows(srow, n_ref, drows, nb, c->w, q.k1, q.k_out, q.rank_off,
                                   d_dist, d_idx, s.st));
     HIP_TRY(c, hipStreamSynchronize(s.st));
   }
@@ -407,10 +537,21 @@ int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const in
 
 namespace {
 
-int query_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int64_t n_fit, int32_t k,
-               uint32_t flags, double* out_dist, int32_t* out_idx, bool out_on_device) {
+// in: query frames (float xyz, nm) or, for a context holding data vectors,
+// FP64 vectors; nullptr = the resident set itself.
+int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, int64_t n_fit,
+               int32_t k, uint32_t flags, double* out_dist, int32_t* out_idx,
+               bool out_on_device) {
   if (!c) return MDSCTK_ERR_ARG;
   if (!c->has_ref) return fail(c, MDSCTK_ERR_STATE, "query: no reference set");
+  if (data != (c->w.metric != kMetricRMSD))
+    return fail(c, MDSCTK_ERR_STATE,
+                data ? "data query: the context holds frames (use mdsctk_knn_query)"
+                     : "query: the context holds data vectors (use mdsctk_data_query)");
+  if (data && xyz && !on_device) {
+    const int rc0 = check_data(c, (const double*)xyz, n_fit, c->ref.vdim, c->w.metric, "query");
+    if (rc0) return rc0;
+  }
   if (xyz == nullptr) n_fit = c->ref.n;
   if (n_fit < 0) return fail(c, MDSCTK_ERR_ARG, "query: negative frame count");
   if (n_fit > INT32_MAX) return fail(c, MDSCTK_ERR_ARG, "query: too many frames");
@@ -430,11 +571,7 @@ int query_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int64_t n_fi
   // Stage (or alias) the query frames and run their prep pass.
   const DevFrames* fitp = &c->ref;
   if (xyz) {
-    if ((rc = slot_frames(c, s, n_fit))) return rc;
-    const size_t xbytes = (size_t)n_fit * c->ref.natoms * 3 * sizeof(float);
-    HIP_TRY(c, hipMemcpyAsync(s.fit.xyz, xyz, xbytes,
-                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s.st));
-    if ((rc = prep_slot(c, s))) return rc;
+    if ((rc = stage_input(c, s, xyz, on_device, n_fit))) return rc;
     fitp = &s.fit;
   }
   HIP_TRY(c, hipEventRecord(s.ev[1], s.st));
@@ -517,6 +654,54 @@ int query_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int64_t n_fi
   return q.k_out;
 }
 
+}  // namespace
+}  // namespace mdsctk
+
+namespace mdsctk {
+namespace {
+
+int debug_scores_impl(mdsctk_knn_ctx* c, const void* xyz_nm, bool data, int64_t n_fit,
+                      uint32_t flags, double* approx_s, double* exact_s) {
+  if (!c) return MDSCTK_ERR_ARG;
+  if (!c->has_ref) return fail(c, MDSCTK_ERR_STATE, "debug_scores: no reference set");
+  if (data != (c->w.metric != kMetricRMSD))
+    return fail(c, MDSCTK_ERR_STATE, "debug_scores: wrong entry point for the resident set");
+  if (data) flags |= MDSCTK_KNN_NOFIT;
+  if (c->prec == kExact) return fail(c, MDSCTK_ERR_STATE, "debug_scores: no contraction form");
+  if (!approx_s) return fail(c, MDSCTK_ERR_ARG, "debug_scores: null output");
+  if (xyz_nm == nullptr) n_fit = c->ref.n;
+  if (n_fit <= 0 || (double)n_fit * c->ref.n > 6.0e7)
+    return fail(c, MDSCTK_ERR_ARG, "debug_scores: n_fit*n_ref must be in (0, 6e7]");
+  HIP_TRY(c, hipSetDevice(c->device));
+  Slot& s = c->main;
+  const bool nofit = flags & MDSCTK_KNN_NOFIT;
+  const DevFrames* fit = &c->ref;
+  int rc;
+  if (xyz_nm) {
+    if ((rc = stage_input(c, s, xyz_nm, false, n_fit))) return rc;
+    fit = &s.fit;
+  }
+  const size_t n = (size_t)n_fit * c->ref.n;
+  void* buf = nullptr;
+  if ((rc = dalloc(c, &buf, n * sizeof(double), "debug scores"))) return rc;
+  Lists L;
+  L.nl = 4;
+  L.cap = 256;
+  hipError_t e = launch_contract(c->ref, *fit, L, nullptr, 1, 1, nofit, (double*)buf,
+                                 std::max(8, c->n_cu & ~7), s.st);  // any grid is valid
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(approx_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, s.st);
+  if (e == hipSuccess && exact_s) {
+    e = launch_exact_rows(c->ref, *fit, c->w, nullptr, (int)n_fit, nofit, (double*)buf, nullptr, 0,
+                          s.st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(exact_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, s.st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s.st);
+  dfree(c, buf);
+  if (e != hipSuccess) return hip_fail(c, e, "debug_scores");
+  return 0;
+}
 }  // namespace
 }  // namespace mdsctk
 
@@ -607,56 +792,41 @@ int mdsctk_knn_set_reference_device(mdsctk_knn_ctx* c, const float* d_xyz_nm, in
 int mdsctk_knn_query(mdsctk_knn_ctx* c, const float* xyz_nm, int64_t n_fit, int32_t k,
                      uint32_t flags, double* out_dist, int32_t* out_idx) {
   if (c && (!out_dist || !out_idx)) return fail(c, MDSCTK_ERR_ARG, "query: null output");
-  return query_impl(c, xyz_nm, false, n_fit, k, flags, out_dist, out_idx, false);
+  return query_impl(c, xyz_nm, false, false, n_fit, k, flags, out_dist, out_idx, false);
 }
 
 int mdsctk_knn_query_device(mdsctk_knn_ctx* c, const float* d_xyz_nm, int64_t n_fit, int32_t k,
                             uint32_t flags, double* d_out_dist, int32_t* d_out_idx) {
   if (c && (!d_out_dist || !d_out_idx)) return fail(c, MDSCTK_ERR_ARG, "query: null output");
-  return query_impl(c, d_xyz_nm, true, n_fit, k, flags, d_out_dist, d_out_idx, true);
+  return query_impl(c, d_xyz_nm, false, true, n_fit, k, flags, d_out_dist, d_out_idx, true);
 }
+
 
 int mdsctk_knn_debug_scores(mdsctk_knn_ctx* c, const float* xyz_nm, int64_t n_fit,
                             uint32_t flags, double* approx_s, double* exact_s) {
-  if (!c) return MDSCTK_ERR_ARG;
-  if (!c->has_ref) return fail(c, MDSCTK_ERR_STATE, "debug_scores: no reference set");
-  if (c->prec == kExact) return fail(c, MDSCTK_ERR_STATE, "debug_scores: no contraction form");
-  if (!approx_s) return fail(c, MDSCTK_ERR_ARG, "debug_scores: null output");
-  if (xyz_nm == nullptr) n_fit = c->ref.n;
-  if (n_fit <= 0 || (double)n_fit * c->ref.n > 6.0e7)
-    return fail(c, MDSCTK_ERR_ARG, "debug_scores: n_fit*n_ref must be in (0, 6e7]");
-  HIP_TRY(c, hipSetDevice(c->device));
-  Slot& s = c->main;
-  const bool nofit = flags & MDSCTK_KNN_NOFIT;
-  const DevFrames* fit = &c->ref;
-  int rc;
-  if (xyz_nm) {
-    if ((rc = slot_frames(c, s, n_fit))) return rc;
-    HIP_TRY(c, hipMemcpyAsync(s.fit.xyz, xyz_nm, (size_t)n_fit * c->ref.natoms * 3 * sizeof(float),
-                              hipMemcpyHostToDevice, s.st));
-    if ((rc = prep_slot(c, s))) return rc;
-    fit = &s.fit;
+  return debug_scores_impl(c, xyz_nm, false, n_fit, flags, approx_s, exact_s);
+}
+
+int mdsctk_data_set_reference(mdsctk_knn_ctx* c, const double* x, int64_t n_ref, int32_t vdim,
+                              int metric) {
+  return set_data_reference_impl(c, x, n_ref, vdim, metric);
+}
+
+int mdsctk_data_query(mdsctk_knn_ctx* c, const double* x, int64_t n_fit, int32_t k,
+                      uint32_t flags, double* out_dist, int32_t* out_idx) {
+  if (c && (!out_dist || !out_idx)) return fail(c, MDSCTK_ERR_ARG, "data query: null output");
+  if (c && (flags & MDSCTK_KNN_NOFIT))
+    return fail(c, MDSCTK_ERR_ARG, "data query: --nofit applies to frames only");
+  return query_impl(c, x, true, false, n_fit, k, flags, out_dist, out_idx, false);
+}
+
+int mdsctk_data_debug_scores(mdsctk_knn_ctx* c, const double* x, int64_t n_fit,
+                             double* approx_s, double* exact_s) {
+  if (c && x && n_fit > 0 && c->has_ref && c->w.metric != kMetricRMSD) {
+    const int rc = check_data(c, x, n_fit, c->ref.vdim, c->w.metric, "debug_scores");
+    if (rc) return rc;
   }
-  const size_t n = (size_t)n_fit * c->ref.n;
-  void* buf = nullptr;
-  if ((rc = dalloc(c, &buf, n * sizeof(double), "debug scores"))) return rc;
-  Lists L;
-  L.nl = 4;
-  L.cap = 256;
-  hipError_t e = launch_contract(c->ref, *fit, L, nullptr, 1, 1, nofit, (double*)buf,
-                                 std::max(8, c->n_cu & ~7), s.st);  // any grid is valid
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(approx_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, s.st);
-  if (e == hipSuccess && exact_s) {
-    e = launch_exact_rows(c->ref, *fit, c->w, nullptr, (int)n_fit, nofit, (double*)buf, nullptr, 0,
-                          s.st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(exact_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, s.st);
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(s.st);
-  dfree(c, buf);
-  if (e != hipSuccess) return hip_fail(c, e, "debug_scores");
-  return 0;
+  return debug_scores_impl(c, x, true, n_fit, 0, approx_s, exact_s);
 }
 
 int mdsctk_knn_get_stats(const mdsctk_knn_ctx* c, mdsctk_knn_stats* out) {

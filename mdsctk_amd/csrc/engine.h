@@ -25,6 +25,8 @@ struct Slot {
   DevFrames fit;
   size_t fit_cap_frames = 0;
   int fit_natoms = 0;
+  int fit_vdim = 0;    // data vectors: length of the staged vectors
+  int fit_metric = 0;
   void* ws_lists = nullptr;
   size_t ws_lists_bytes = 0;
   void* ws_small = nullptr;
@@ -75,8 +77,9 @@ struct mdsctk_knn_ctx {
   int serve_slots = 2;
   int64_t serve_max_frames = 8192;
 
-  mdsctk::DevWeights w;
+  mdsctk::DevWeights w;  // masses, or unit masses plus the metric for data vectors
   mdsctk::DevFrames ref;
+  double* data_shift = nullptr;  // data vectors, Euclidean: the reference set's mean vector
   bool has_ref = false;
   mdsctk::Slot main;  // workspace of the synchronous entry points
   mdsctk_knn_stats stats = {};

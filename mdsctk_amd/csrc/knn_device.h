@@ -18,6 +18,17 @@ constexpr int kTileAlign = 4;
 
 enum Prec : int { kFP32 = 0, kBF16 = 1, kFP16 = 2, kExact = 3 };
 
+// What a frame is and which distance the engine ranks by.
+//   kMetricRMSD         frames of atoms: optimal-superposition RMSD (knn_rms)
+//   kMetricEuclidean    data vectors: Euclidean distance (knn_data)
+//   kMetricCorrelation  data vectors: correlation distance sqrt((1-r)/2)
+//                       (knn_data -c)
+// Data vectors run through the same contraction as frames with the QCP
+// step replaced by the --nofit trace: a vector of length V is laid out as
+// ceil(V/3) "atoms" of 3 values with unit masses, uncentred, so that the
+// trace of the 3x3 block is the dot product.
+enum Metric : int { kMetricRMSD = 0, kMetricEuclidean = 1, kMetricCorrelation = 2 };
+
 // Atoms per operand K-group: split forms hold 16 atoms (one 32x32x16 MFMA
 // step) per group, FP32 holds 8 atoms (four 32x32x2 MFMA steps).
 inline int group_atoms(int prec) { return prec == kFP32 ? 8 : 16; }
@@ -43,13 +54,23 @@ struct DevFrames {
   double* Gt = nullptr;   // [n_tiles*32] same sum over the rounded operand
   uint4* op = nullptr;    // contraction operand, fragment-major (see prep kernel)
   bool owns_xyz = true;
+  // data vectors only: the FP64 vectors [n][vdim] the exact scores use and,
+  // for the correlation metric, [n][2] (mean, sample standard deviation)
+  int vdim = 0;
+  double* x64 = nullptr;
+  double* aux = nullptr;
 };
 
 struct DevWeights {
   int natoms = 0;
   float* w = nullptr;   // masses
   float* sw = nullptr;  // sqrt(mass)
-  double W = 0;         // sum of masses
+  // Output distance = out_scale * sqrt(s / W): for frames W is the sum of
+  // masses and out_scale 10 (nm to Angstrom); for data vectors W is 1
+  // (Euclidean) or 4 (correlation, s = 2(1-r)) and out_scale 1.
+  double W = 0;
+  double out_scale = 10.0;
+  int metric = kMetricRMSD;
 };
 
 // Per-query candidate lists written by the contraction kernel.
@@ -87,9 +108,14 @@ hipError_t launch_exact_rows(const DevFrames& ref, const DevFrames& fit, const D
 // Top-k1 by (s, index) over exact score rows; writes ranks
 // [rank_off, rank_off+k_out) of each listed fit row.
 hipError_t launch_select_rows(const double* srow, int64_t n_ref, const int* rows, int nrows,
-                              double W, int k1, int k_out, int rank_off, double* out_dist,
-                              int* out_idx, hipStream_t st);
+                              const DevWeights& w, int k1, int k_out, int rank_off,
+                              double* out_dist, int* out_idx, hipStream_t st);
 hipError_t launch_identity_rows(int64_t n_fit, int64_t n_ref, int* out_idx, hipStream_t st);
+// Data vectors: mean[e] over the n vectors x[n][vdim] (FP64).
+hipError_t launch_column_mean(const double* x, int64_t n, int vdim, double* mean, hipStream_t st);
+// Data vectors to the frame layout of f (f.xyz, f.natoms * 3 >= f.vdim):
+// Euclidean x - shift, correlation (x - mean) / |x - mean| with f.aux.
+hipError_t launch_data_prep(const DevFrames& f, int metric, const double* shift, hipStream_t st);
 hipError_t launch_max_gt(const DevFrames& f, double* out, hipStream_t st);
 
 // Fit tiles per contraction workgroup for a precision (2 or 4).

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz
                                                    int natoms, int n_groups,
                                                    const float* __restrict__ w,
                                                    const float* __restrict__ sw, double Wtot,
-                                                   double* __restrict__ cen,
+                                                   int center, double* __restrict__ cen,
                                                    double* __restrict__ G,
                                                    double* __restrict__ Gt,
                                                    uint4* __restrict__ op) {
@@ -252,8 +252,10 @@ __global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz
   };
   auto local_atom = [&](int j) { return (PREC == kFP32) ? (2 * j + h) : (8 * h + j); };
 
+  // first pass: the mass-weighted centroid (skipped for data vectors, which
+  // are used as given)
   double acc = 0.0;
-  for (int g = 0; g < n_groups; ++g) {
+  for (int g = 0; center && g < n_groups; ++g) {
     __syncthreads();
     stage(g);
     __syncthreads();
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz
   __syncthreads();
   if (t < 96) {
     const int cc = t >> 5, rr = t & 31;
-    scen[rr * 3 + cc] = (red[cc * 64 + rr] + red[cc * 64 + 32 + rr]) / Wtot;
+    scen[rr * 3 + cc] = center ? (red[cc * 64 + rr] + red[cc * 64 + 32 + rr]) / Wtot : 0.0;
   }
   __syncthreads();
   const double cm = scen[row * 3 + c];
@@ -763,10 +765,38 @@ __device__ __forceinline__ double wave_exact_score(const float* __restrict__ a,
   return fmax(ga + gb - 2.0 * lam, 0.0);
 }
 
-// ---------------------------------------------------------------------------
-// merge: radix select over the lists, certified window, FP64 re-score.
+// Exact FP64 score of one pair of data vectors (a: reference, b: query),
+// lanes over elements: the squared Euclidean distance
+// (euclidean_distance, mdsctk.cpp:329-334), or for the correlation metric
+// s = 4 max(v, 0) with v = (1 - cov / ((V-1) sd_b sd_a)) / 2 as
+// correlation_distance forms it (mdsctk.cpp:335-356; its first argument is
+// the query vector).
+// The output distance is then sqrt(s) or sqrt(s / 4) = sqrt(v).
+__device__ __forceinline__ double wave_data_score(const double* __restrict__ a,
+                                                  const double* __restrict__ a_aux,
+                                                  const double* __restrict__ b,
+                                                  const double* __restrict__ b_aux, int vdim,
+                                                  int metric) {
+  const int lane = threadIdx.x & 63;
+  double acc = 0.0;
+  if (metric == kMetricEuclidean) {
+    for (int e = lane; e < vdim; e += 64) {
+      const double d = b[e] - a[e];
+      acc += d * d;
+    }
+    return wave_sum_d(acc);
+  }
+  const double ma = a_aux[0], mb = b_aux[0];
+  for (int e = lane; e < vdim; e += 64) acc += (b[e] - mb) * (a[e] - ma);
+  acc = wave_sum_d(acc);
+  const double v = (1.0 - (acc / ((((double)vdim - 1.0) * b_aux[1]) * a_aux[1]))) / 2.0;
+  // clamped as the reference clamps v: scores are never negative, which the
+  // exact-row selection's 64-bit key order relies on
+  return fmax(4.0 * v, 0.0);
+}
 
-struct MergeParams {
+// The exact score of reference r against query q under the set's metric.
+struct PairSource {
   const float* rxyz;
   const double* rcen;
   const double* rG;
@@ -775,7 +805,29 @@ struct MergeParams {
   const double* fG;
   const float* w;
   int natoms;
-  double W;
+  int metric, vdim;
+  const double* r64;
+  const double* raux;
+  const double* f64;
+  const double* faux;
+};
+
+__device__ __forceinline__ double wave_pair_score(const PairSource& P, int64_t r, int64_t q,
+                                                  bool nofit) {
+  if (P.metric == kMetricRMSD)
+    return wave_exact_score(P.rxyz + r * (int64_t)P.natoms * 3, P.rcen + r * 3, P.rG[r],
+                            P.fxyz + q * (int64_t)P.natoms * 3, P.fcen + q * 3, P.fG[q], P.w,
+                            P.natoms, nofit);
+  return wave_data_score(P.r64 + r * (int64_t)P.vdim, P.raux + r * 2, P.f64 + q * (int64_t)P.vdim,
+                         P.faux + q * 2, P.vdim, P.metric);
+}
+
+// ---------------------------------------------------------------------------
+// merge: radix select over the lists, certified window, FP64 re-score.
+
+struct MergeParams {
+  PairSource src;
+  double W, out_scale;
   const uint2* ent;
   const int* cnt;
   const float* ovf;
@@ -888,11 +940,8 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
     atomicAdd(P.n_cand, nc);
   }
 
-  const float* fx = P.fxyz + q * (int64_t)P.natoms * 3;
   for (int c = wave; c < nc; c += 4) {
-    const int64_t r = cand_idx[c];
-    const double s = wave_exact_score(P.rxyz + r * (int64_t)P.natoms * 3, P.rcen + r * 3, P.rG[r],
-                                      fx, P.fcen + q * 3, P.fG[q], P.w, P.natoms, P.nofit);
+    const double s = wave_pair_score(P.src, cand_idx[c], q, P.nofit);
     if (lane == 0) cand_s[c] = s;
   }
   int np2 = 64;
@@ -904,7 +953,7 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
   block_bitonic_sort(cand_s, cand_idx, np2);
   for (int j = tid; j < P.k_out; j += 256) {
     const int src = j + P.rank_off;
-    P.out_dist[q * P.k_out + j] = 10.0 * sqrt(fmax(cand_s[src], 0.0) / P.W);
+    P.out_dist[q * P.k_out + j] = P.out_scale * sqrt(fmax(cand_s[src], 0.0) / P.W);
     P.out_idx[q * P.k_out + j] = cand_idx[src];
   }
 }
@@ -913,15 +962,8 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
 // exact rows
 
 struct ExactParams {
-  const float* rxyz;
-  const double* rcen;
-  const double* rG;
-  const float* fxyz;
-  const double* fcen;
-  const double* fG;
-  const float* w;
-  int natoms;
-  double W;
+  PairSource src;
+  double W, out_scale;
   int64_t n_ref;
   const int* rows;
   int64_t row0;
@@ -937,16 +979,14 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(ExactParams P) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t ri = blockIdx.y;
   const int64_t q = P.rows ? P.rows[ri] : P.row0 + ri;
-  const float* fx = P.fxyz + q * (int64_t)P.natoms * 3;
   const int64_t r0 = (int64_t)blockIdx.x * kExactRefsPerWG;
   for (int j = wave; j < kExactRefsPerWG; j += 4) {
     const int64_t r = r0 + j;
     if (r >= P.n_ref) break;
-    const double s = wave_exact_score(P.rxyz + r * (int64_t)P.natoms * 3, P.rcen + r * 3, P.rG[r],
-                                      fx, P.fcen + q * 3, P.fG[q], P.w, P.natoms, P.nofit);
+    const double s = wave_pair_score(P.src, r, q, P.nofit);
     if (lane == 0) {
       if (P.out_full)
-        P.out_full[q * P.out_ld + r] = 10.0 * sqrt(s / P.W);
+        P.out_full[q * P.out_ld + r] = P.out_scale * sqrt(fmax(s, 0.0) / P.W);
       else
         P.srow[ri * P.n_ref + r] = s;
     }
@@ -957,7 +997,8 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(ExactParams P) {
 __global__ __launch_bounds__(256) void select_rows_kernel(const double* __restrict__ srow,
                                                           int64_t n_ref,
                                                           const int* __restrict__ rows,
-                                                          double W, int k1, int k_out,
+                                                          double W, double out_scale,
+                                                          int k1, int k_out,
                                                           int rank_off, double* out_dist,
                                                           int* out_idx) {
   const int64_t ri = blockIdx.x;
@@ -1048,7 +1089,7 @@ __global__ __launch_bounds__(256) void select_rows_kernel(const double* __restri
   block_bitonic_sort(cand_s, cand_idx, np2);
   for (int j = tid; j < k_out; j += 256) {
     const int src = j + rank_off;
-    out_dist[q * k_out + j] = 10.0 * sqrt(fmax(cand_s[src], 0.0) / W);
+    out_dist[q * k_out + j] = out_scale * sqrt(fmax(cand_s[src], 0.0) / W);
     out_idx[q * k_out + j] = cand_idx[src];
   }
 }
@@ -1087,6 +1128,66 @@ __global__ void identity_kernel(int64_t n_fit, int64_t n_ref, int* out) {
   if (i < n_fit * n_ref) out[i] = (int)(i % n_ref);
 }
 
+
+// ---------------------------------------------------------------------------
+// data vectors (knn_data)
+
+// mean[e] = (sum over the n vectors of x[i][e]) / n, one thread per element.
+__global__ __launch_bounds__(256) void column_mean_kernel(const double* __restrict__ x, int64_t n,
+                                                          int vdim, double* __restrict__ mean) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= vdim) return;
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += x[i * vdim + e];
+  mean[e] = s / (double)n;
+}
+
+// One wave per vector: the float source values of the contraction operand,
+// laid out as natoms "atoms" of 3 values (zero beyond vdim).  Euclidean:
+// x - shift (any common shift leaves the distances unchanged and a centred
+// set has smaller G terms, so a tighter window).  Correlation: the unit
+// vector (x - mean) / |x - mean|, whose squared distances are 2(1 - r); aux
+// gets the mean and the sample standard deviation in the form
+// correlation_distance computes it (mdsctk.cpp:335-349).
+__global__ __launch_bounds__(256) void data_prep_kernel(const double* __restrict__ x, int64_t n,
+                                                        int vdim, int natoms, int metric,
+                                                        const double* __restrict__ shift,
+                                                        float* __restrict__ out,
+                                                        double* __restrict__ aux) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+  if (row >= n) return;
+  const double* xr = x + row * vdim;
+  float* o = out + row * (int64_t)natoms * 3;
+  const int width = natoms * 3;
+  if (metric == kMetricCorrelation) {
+    double s = 0.0, ss = 0.0;
+    for (int e = lane; e < vdim; e += 64) {
+      const double v = xr[e];
+      s += v;
+      ss += v * v;
+    }
+    s = wave_sum_d(s);
+    ss = wave_sum_d(ss);
+    const double dn = (double)vdim;
+    const double mean = s / dn;
+    double c2 = 0.0;
+    for (int e = lane; e < vdim; e += 64) {
+      const double d = xr[e] - mean;
+      c2 += d * d;
+    }
+    c2 = wave_sum_d(c2);
+    const double inv = c2 > 0.0 ? 1.0 / sqrt(c2) : 0.0;
+    for (int e = lane; e < width; e += 64) o[e] = e < vdim ? (float)((xr[e] - mean) * inv) : 0.f;
+    if (lane == 0) {
+      aux[row * 2 + 0] = mean;
+      aux[row * 2 + 1] = sqrt(((dn * ss) - (s * s)) / (dn * (dn - 1.0)));
+    }
+  } else {
+    for (int e = lane; e < width; e += 64) o[e] = e < vdim ? (float)(xr[e] - shift[e]) : 0.f;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1103,18 +1204,19 @@ int list_capacity_for(int k1) {
 
 hipError_t launch_prep(const DevFrames& f, const DevWeights& w, hipStream_t st) {
   dim3 grid(f.n_tiles), block(192);
+  const int center = w.metric == kMetricRMSD ? 1 : 0;
   switch (f.prec) {
     case kFP32:
       hipLaunchKernelGGL(prep_kernel<kFP32>, grid, block, 0, st, f.xyz, f.n, f.natoms,
-                         f.n_groups, w.w, w.sw, w.W, f.cen, f.G, f.Gt, f.op);
+                         f.n_groups, w.w, w.sw, w.W, center, f.cen, f.G, f.Gt, f.op);
       break;
     case kBF16:
       hipLaunchKernelGGL(prep_kernel<kBF16>, grid, block, 0, st, f.xyz, f.n, f.natoms,
-                         f.n_groups, w.w, w.sw, w.W, f.cen, f.G, f.Gt, f.op);
+                         f.n_groups, w.w, w.sw, w.W, center, f.cen, f.G, f.Gt, f.op);
       break;
     case kFP16:
       hipLaunchKernelGGL(prep_kernel<kFP16>, grid, block, 0, st, f.xyz, f.n, f.natoms,
-                         f.n_groups, w.w, w.sw, w.W, f.cen, f.G, f.Gt, f.op);
+                         f.n_groups, w.w, w.sw, w.W, center, f.cen, f.G, f.Gt, f.op);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1200,11 +1302,8 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   return hipGetLastError();
 }
 
-hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
-                        const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
-                        bool nofit, double* out_dist, int* out_idx, int* fallback, int* n_cand,
-                        hipStream_t st) {
-  MergeParams P;
+static PairSource pair_source(const DevFrames& ref, const DevFrames& fit, const DevWeights& w) {
+  PairSource P;
   P.rxyz = ref.xyz;
   P.rcen = ref.cen;
   P.rG = ref.G;
@@ -1213,7 +1312,23 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
   P.fG = fit.G;
   P.w = w.w;
   P.natoms = ref.natoms;
+  P.metric = w.metric;
+  P.vdim = ref.vdim;
+  P.r64 = ref.x64;
+  P.raux = ref.aux;
+  P.f64 = fit.x64;
+  P.faux = fit.aux;
+  return P;
+}
+
+hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
+                        const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
+                        bool nofit, double* out_dist, int* out_idx, int* fallback, int* n_cand,
+                        hipStream_t st) {
+  MergeParams P;
+  P.src = pair_source(ref, fit, w);
   P.W = w.W;
+  P.out_scale = w.out_scale;
   P.ent = L.ent;
   P.cnt = L.cnt;
   P.ovf = L.ovf;
@@ -1236,15 +1351,9 @@ hipError_t launch_exact_rows(const DevFrames& ref, const DevFrames& fit, const D
                              const int* rows, int nrows, bool nofit, double* srow,
                              double* out_full, int64_t out_ld, hipStream_t st) {
   ExactParams P;
-  P.rxyz = ref.xyz;
-  P.rcen = ref.cen;
-  P.rG = ref.G;
-  P.fxyz = fit.xyz;
-  P.fcen = fit.cen;
-  P.fG = fit.G;
-  P.w = w.w;
-  P.natoms = ref.natoms;
+  P.src = pair_source(ref, fit, w);
   P.W = w.W;
+  P.out_scale = w.out_scale;
   P.n_ref = ref.n;
   P.nofit = nofit ? 1 : 0;
   P.out_ld = out_ld;
@@ -1263,11 +1372,26 @@ hipError_t launch_exact_rows(const DevFrames& ref, const DevFrames& fit, const D
 }
 
 hipError_t launch_select_rows(const double* srow, int64_t n_ref, const int* rows, int nrows,
-                              double W, int k1, int k_out, int rank_off, double* out_dist,
-                              int* out_idx, hipStream_t st) {
+                              const DevWeights& w, int k1, int k_out, int rank_off,
+                              double* out_dist, int* out_idx, hipStream_t st) {
   if (k1 > kCandMax) return hipErrorInvalidValue;
   hipLaunchKernelGGL(select_rows_kernel, dim3((unsigned)nrows), dim3(256), 0, st, srow, n_ref,
-                     rows, W, k1, k_out, rank_off, out_dist, out_idx);
+                     rows, w.W, w.out_scale, k1, k_out, rank_off, out_dist, out_idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_column_mean(const double* x, int64_t n, int vdim, double* mean, hipStream_t st) {
+  hipLaunchKernelGGL(column_mean_kernel, dim3((unsigned)((vdim + 255) / 256)), dim3(256), 0, st, x,
+                     n, vdim, mean);
+  return hipGetLastError();
+}
+
+hipError_t launch_data_prep(const DevFrames& f, int metric, const double* shift, hipStream_t st) {
+  if (f.vdim <= 0 || f.natoms * 3 < f.vdim || !f.x64 || !f.xyz) return hipErrorInvalidValue;
+  if (metric == kMetricCorrelation ? !f.aux : !shift) return hipErrorInvalidValue;
+  if (f.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(data_prep_kernel, dim3((unsigned)((f.n + 3) / 4)), dim3(256), 0, st, f.x64,
+                     f.n, f.vdim, f.natoms, metric, shift, f.xyz, f.aux);
   return hipGetLastError();
 }
 

@@ -522,6 +522,9 @@ int server_submit(mdsctk_knn_ctx* c, const float* xyz, int64_t n_fit, int32_t k,
   *id = 0;
   std::lock_guard<std::mutex> g(c->server_mu);
   if (!c->has_ref) return fail(c, MDSCTK_ERR_STATE, "submit: no reference set");
+  if (c->w.metric != kMetricRMSD)
+    return fail(c, MDSCTK_ERR_STATE,
+                "submit: serving takes frames; data vectors use mdsctk_data_query");
   if (!xyz || n_fit < 0 || n_fit > INT32_MAX || (n_fit > 0 && (!out_dist || !out_idx)))
     return fail(c, MDSCTK_ERR_ARG, "submit: need query frames and outputs (n_fit=%lld)",
                 (long long)n_fit);

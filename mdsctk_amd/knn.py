@@ -24,6 +24,10 @@ KNN_NOFIT = 1
 KNN_FULL_ROWS = 2
 KNN_KEEP_RANK0 = 4
 
+METRIC_EUCLIDEAN = 1
+METRIC_CORRELATION = 2
+METRICS = {"euclidean": METRIC_EUCLIDEAN, "correlation": METRIC_CORRELATION}
+
 OPT_HBM_BUDGET = 1
 OPT_EPS_SCALE = 2
 OPT_REF_SPLITS = 3
@@ -141,6 +145,12 @@ def load():
     L.mdsctk_knn_serve_stats_get.restype = ctypes.c_int
     L.mdsctk_knn_clock_ms.argtypes = [vp]
     L.mdsctk_knn_clock_ms.restype = dbl
+    L.mdsctk_data_set_reference.argtypes = [vp, vp, i64, i32, ctypes.c_int]
+    L.mdsctk_data_set_reference.restype = ctypes.c_int
+    L.mdsctk_data_query.argtypes = [vp, vp, i64, i32, u32, vp, vp]
+    L.mdsctk_data_query.restype = ctypes.c_int
+    L.mdsctk_data_debug_scores.argtypes = [vp, vp, i64, vp, vp]
+    L.mdsctk_data_debug_scores.restype = ctypes.c_int
     _bind_io(L)
     _lib = L
     return L
@@ -263,6 +273,41 @@ class KnnContext:
         a = np.zeros((n_fit, self.n_ref), np.float64)
         e = np.zeros((n_fit, self.n_ref), np.float64) if exact else None
         self._check(self._L.mdsctk_knn_debug_scores(self._h, xp, n_fit, flags, _ptr(a), _ptr(e)))
+        return a, e
+
+    # -- knn_data: data vectors ------------------------------------------------
+
+    def set_data_reference(self, x, metric="euclidean"):
+        """Resident FP64 data vectors x[n_ref, vdim] (knn_data)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n, vdim = x.shape
+        self._check(self._L.mdsctk_data_set_reference(self._h, _ptr(x), n, vdim, METRICS[metric]))
+        self.n_ref, self.vdim = n, vdim
+
+    def data_query(self, x=None, k=0, flags=0):
+        """knn_data rows: (dist[n_fit, width], idx[n_fit, width]); x None = self."""
+        if x is None:
+            n_fit, xp = self.n_ref, None
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            assert x.shape[1] == self.vdim
+            n_fit, xp = x.shape[0], _ptr(x)
+        width = self.row_width(k, flags)
+        d = np.zeros((n_fit, width), np.float64)
+        i = np.zeros((n_fit, width), np.int32)
+        got = self._check(self._L.mdsctk_data_query(self._h, xp, n_fit, k, flags, _ptr(d), _ptr(i)))
+        assert got == width
+        return d, i
+
+    def data_debug_scores(self, x=None, exact=True):
+        if x is None:
+            n_fit, xp = self.n_ref, None
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            n_fit, xp = x.shape[0], _ptr(x)
+        a = np.zeros((n_fit, self.n_ref), np.float64)
+        e = np.zeros((n_fit, self.n_ref), np.float64) if exact else None
+        self._check(self._L.mdsctk_data_debug_scores(self._h, xp, n_fit, _ptr(a), _ptr(e)))
         return a, e
 
     def stats(self) -> dict:

@@ -101,3 +101,37 @@ def masses(natoms: int, kind: str = "uniform", seed: int = 0) -> np.ndarray:
     if kind == "random":
         return np.random.default_rng(seed).uniform(1.0, 32.0, natoms).astype(np.float32)
     raise ValueError(kind)
+
+
+def data_vectors(n: int, vdim: int, seed: int, kind: str = "clusters",
+                 n_centres: int = 20) -> np.ndarray:
+    """Seeded FP64 data vectors [n, vdim] of the kinds knn_data reads from
+    .pts files.
+
+      clusters   n_centres Gaussian centres N(0, 1) plus N(0, 0.3^2) noise,
+                 with 5 % near-duplicates (an earlier vector plus N(0, 1e-6^2))
+      dihedral   sin/cos pairs of angles around n_centres conformations
+                 (the angles_to_sincos form), noise 0.3 rad
+      offset     clusters shifted by 50 in every element (a large common
+                 component the correlation metric removes and the Euclidean
+                 one must not lose precision on)
+    """
+    rng = np.random.default_rng(seed)
+    if kind == "dihedral":
+        half = (vdim + 1) // 2
+        centres = rng.uniform(-np.pi, np.pi, (n_centres, half))
+        ang = centres[rng.integers(0, n_centres, n)] + rng.normal(0.0, 0.3, (n, half))
+        out = np.empty((n, 2 * half))
+        out[:, 0::2] = np.sin(ang)
+        out[:, 1::2] = np.cos(ang)
+        return np.ascontiguousarray(out[:, :vdim])
+    centres = rng.normal(0.0, 1.0, (n_centres, vdim))
+    x = centres[rng.integers(0, n_centres, n)] + rng.normal(0.0, 0.3, (n, vdim))
+    dup = np.nonzero(rng.random(n) < 0.05)[0]
+    dup = dup[dup > 0]
+    if len(dup):
+        src = (rng.random(len(dup)) * dup).astype(np.int64)
+        x[dup] = x[src] + rng.normal(0.0, 1e-6, (len(dup), vdim))
+    if kind == "offset":
+        x += 50.0
+    return x

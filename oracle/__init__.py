@@ -57,6 +57,13 @@ def lib():
         L.orc_port_knn_rows.argtypes = [f32p, c_i64, f32p, c_i64, c_i64, c_int, f32p, c_int,
                                         c_int, c_int, f64p, i32p, c_int]
         L.orc_port_knn_rows.restype = c_int
+        L.orc_euclidean.argtypes = [c_int, f64p, f64p]
+        L.orc_euclidean.restype = c_dbl
+        L.orc_correlation.argtypes = [c_int, f64p, f64p]
+        L.orc_correlation.restype = c_dbl
+        L.orc_knn_data.argtypes = [f64p, c_i64, f64p, c_i64, c_int, c_int, c_int, c_int, f64p,
+                                   i32p, c_int]
+        L.orc_knn_data.restype = c_int
         _lib = L
     return _lib
 
@@ -149,4 +156,28 @@ def port_knn_rows(ref_c, fit_c, w, k, row0=0, row1=None, sort=True, nofit=False,
     i = np.zeros((n, width), np.int32)
     lib().orc_port_knn_rows(ref_c.reshape(-1), n_ref, fit_c.reshape(-1), row0, row1,
                             ref_c.shape[1], _f32(w), k, int(sort), int(nofit), d, i, nthreads)
+    return d, i
+
+
+METRICS = {"euclidean": 1, "correlation": 2}
+
+
+def data_distance(a, b, metric="euclidean"):
+    """One knn_data distance, query vector a against reference vector b."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    f = lib().orc_euclidean if metric == "euclidean" else lib().orc_correlation
+    return f(a.shape[0], a, b)
+
+
+def knn_data(ref, fit, k, metric="euclidean", keep_rank0=False, nthreads=0):
+    """FP64 knn_data in the reference's file layout: (dist[n_fit,k], idx[n_fit,k])."""
+    ref = np.ascontiguousarray(ref, np.float64)
+    fit = np.ascontiguousarray(fit, np.float64)
+    kk = max(min(k, ref.shape[0] - 1), 0)
+    d = np.zeros((fit.shape[0], kk), np.float64)
+    i = np.zeros((fit.shape[0], kk), np.int32)
+    kk2 = lib().orc_knn_data(ref.reshape(-1), ref.shape[0], fit.reshape(-1), fit.shape[0],
+                             ref.shape[1], METRICS[metric], k, int(keep_rank0), d, i, nthreads)
+    assert kk2 == kk
     return d, i

@@ -35,6 +35,13 @@
 //     independent std::partial_sort calls (mdsctk.h:177-199).  OpenMP over
 //     fit frames (knn_rms.cpp:232,268).  This is the timed CPU baseline.
 //     Pinned against the FP64 oracle (agreement to float accuracy).
+//
+//  3. knn_data (orc_knn_data, orc_euclidean, orc_correlation): the
+//     reference's FP64 distance functions (mdsctk.cpp:329-356) with their
+//     operations in the same order, and knn_data's top-(k+1) / drop-rank-0
+//     output (knn_data.cpp:196-237).  Pinned by numpy closed forms in
+//     tests/test_oracle.py: Euclidean norms of differences, and
+//     sqrt((1 - r) / 2) with r from numpy.corrcoef.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -452,6 +459,71 @@ int orc_knn_exact(const float* ref, int64_t n_ref, const float* fit, int64_t n_f
       int off = keep_rank0 ? 0 : 1;
       for (int j = 0; j < k; ++j) {
         out_A[(size_t)f * k + j] = row[j + off].first;
+        out_idx[(size_t)f * k + j] = row[j + off].second;
+      }
+    }
+  }
+  return k;
+}
+
+// ---- knn_data: data vectors --------------------------------------------------
+
+// euclidean_distance (mdsctk.cpp:329-334), the same operations in the same
+// order; knn_data passes the query vector first (knn_data.cpp:206-209).
+double orc_euclidean(int size, const double* reference, const double* fitting) {
+  double value = 0.0;
+  for (int x = 0; x < size; x++) value += (reference[x] - fitting[x]) * (reference[x] - fitting[x]);
+  return std::sqrt(value);
+}
+
+// correlation_distance (mdsctk.cpp:335-356): sums, the sample standard
+// deviations in the reference's one-pass form, the centred product, then
+// sqrt(max(0, (1 - r) / 2)).
+double orc_correlation(int size, const double* reference, const double* fitting) {
+  double rvalue = 0.0, rsvalue = 0.0, fvalue = 0.0, fsvalue = 0.0;
+  const double dsize = (double)size;
+  double value = 0.0;
+  for (int x = 0; x < size; x++) {
+    rvalue += reference[x];
+    rsvalue += reference[x] * reference[x];
+    fvalue += fitting[x];
+    fsvalue += fitting[x] * fitting[x];
+  }
+  rsvalue = std::sqrt(((dsize * rsvalue) - (rvalue * rvalue)) / (dsize * (dsize - 1.0)));
+  rvalue /= dsize;
+  fsvalue = std::sqrt(((dsize * fsvalue) - (fvalue * fvalue)) / (dsize * (dsize - 1.0)));
+  fvalue /= dsize;
+  for (int x = 0; x < size; x++) value += (reference[x] - rvalue) * (fitting[x] - fvalue);
+  value = (1.0 - (value / ((dsize - 1.0) * rsvalue * fsvalue))) / 2.0;
+  if (value < 0.0) value = 0.0;
+  return std::sqrt(value);
+}
+
+// knn_data's pair loop and output (knn_data.cpp:196-237): per query row the
+// k+1 smallest distances ordered by (value, index), ranks 1..k written (or
+// 0..k-1 with keep_rank0); k clamped to n_ref-1 with k+1 recomputed.
+// metric 1 = Euclidean, 2 = correlation.  Returns the k used.
+int orc_knn_data(const double* ref, int64_t n_ref, const double* fit, int64_t n_fit, int vdim,
+                 int metric, int k, int keep_rank0, double* out, int* out_idx, int nthreads) {
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+  if (n_ref <= 0 || (metric != 1 && metric != 2)) return -1;
+  if (k > n_ref - 1) k = (int)(n_ref - 1);
+  if (k < 0) k = 0;
+  const int k1 = k + 1;
+#pragma omp parallel num_threads(nthreads)
+  {
+    std::vector<std::pair<double, int>> row(n_ref);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t f = 0; f < n_fit; ++f) {
+      const double* q = fit + (size_t)f * vdim;
+      for (int64_t r = 0; r < n_ref; ++r) {
+        const double* x = ref + (size_t)r * vdim;
+        row[r] = {metric == 1 ? orc_euclidean(vdim, q, x) : orc_correlation(vdim, q, x), (int)r};
+      }
+      std::partial_sort(row.begin(), row.begin() + k1, row.end());
+      const int off = keep_rank0 ? 0 : 1;
+      for (int j = 0; j < k; ++j) {
+        out[(size_t)f * k + j] = row[j + off].first;
         out_idx[(size_t)f * k + j] = row[j + off].second;
       }
     }

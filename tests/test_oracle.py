@@ -110,3 +110,43 @@ def test_port_full_rows_layout():
     assert d.shape == (9, 9) and (i == np.arange(9)[None]).all()
     full = oracle.full_exact(ref, ref, w)
     assert np.abs(d - full).max() < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# knn_data distances (mdsctk.cpp:329-356)
+
+
+@pytest.mark.parametrize("kind", ["clusters", "dihedral", "offset"])
+def test_data_distances_closed_forms(kind):
+    from scipy.spatial.distance import cdist
+    x = synth.data_vectors(60, 37, 3, kind)
+    y = synth.data_vectors(25, 37, 4, kind)
+    e = np.array([[oracle.data_distance(a, b) for b in x] for a in y])
+    assert np.abs(e - cdist(y, x)).max() <= 1e-12 * cdist(y, x).max()
+    c = np.array([[oracle.data_distance(a, b, "correlation") for b in x] for a in y])
+    r = np.corrcoef(np.vstack([y, x]))[:len(y), len(y):]
+    assert np.abs(c - np.sqrt(np.maximum((1.0 - r) / 2.0, 0.0))).max() < 1e-9
+    # a vector against itself: 0; against its negation: correlation 1
+    assert oracle.data_distance(x[0], x[0]) == 0.0
+    assert abs(oracle.data_distance(x[0], -x[0], "correlation") - 1.0) < 1e-12
+    # correlation ignores scale and offset
+    assert abs(oracle.data_distance(x[0], 3.0 * x[1] + 7.0, "correlation")
+               - oracle.data_distance(x[0], x[1], "correlation")) < 1e-12
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "correlation"])
+def test_knn_data_rows(metric):
+    from scipy.spatial.distance import cdist
+    x = synth.data_vectors(300, 24, 5)
+    y = synth.data_vectors(40, 24, 6)
+    full = cdist(y, x) if metric == "euclidean" else np.sqrt(np.maximum(
+        (1.0 - np.corrcoef(np.vstack([y, x]))[:40, 40:]) / 2.0, 0.0))
+    d, i = oracle.knn_data(x, y, 7, metric)
+    order = np.lexsort((np.broadcast_to(np.arange(300), full.shape), full), axis=1)
+    assert (i == order[:, 1:8]).all()
+    assert np.abs(d - np.take_along_axis(full, order[:, 1:8], 1)).max() < 1e-9
+    d0, i0 = oracle.knn_data(x, y, 7, metric, keep_rank0=True)
+    assert (i0 == order[:, :7]).all()
+    # k clamps to n_ref - 1
+    d, i = oracle.knn_data(x[:5], y, 30, metric)
+    assert d.shape == (40, 4)
