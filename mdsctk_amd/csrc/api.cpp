@@ -501,7 +501,7 @@ int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist
   HIP_TRY(c, launch_contract(c->ref, fit, L, eps2, q.k1, phases, q.nofit, nullptr, grid, s.st));
   HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
   HIP_TRY(c, launch_merge(c->ref, fit, c->w, L, eps2, q.k1, q.k_out, q.rank_off, q.nofit, d_dist,
-                          d_idx, d_fb, s.n_cand, s.st));
+                          d_idx, d_fb, s.n_cand, nullptr, s.st));
   HIP_TRY(c, hipEventRecord(s.ev[4], s.st));
   info->phases = phases;
   info->cap = cap;

@@ -94,10 +94,12 @@ hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, 
 hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lists& L,
                            const float* eps2, int k1, int ref_phases, bool nofit,
                            double* dbg_scores, int grid_wgs, hipStream_t st);
+// order: the query rows in the order the merge workgroups take them
+// (nullptr: 0..n_fit-1).
 hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
                         const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
                         bool nofit, double* out_dist, int* out_idx, int* fallback,
-                        int* n_cand, hipStream_t st);
+                        int* n_cand, const int* order, hipStream_t st);
 // Exact FP64 scores s = G_a + G_b - 2 lambda for fit rows [row0,row0+nrows)
 // against all refs.  Writes s into srow[(row-row0)*n_ref + ref] unless
 // out_full is given, in which case it writes Angstrom distances into

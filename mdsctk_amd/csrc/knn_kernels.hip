@@ -736,24 +736,38 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
 // ---------------------------------------------------------------------------
 // exact FP64 score of one pair, cooperatively by a wave (lanes over atoms).
 
-__device__ __forceinline__ double wave_exact_score(const float* __restrict__ a,
-                                                   const double* __restrict__ ca, double ga,
-                                                   const float* __restrict__ b,
-                                                   const double* __restrict__ cb, double gb,
-                                                   const float* __restrict__ w, int natoms,
-                                                   bool nofit) {
+// One atom's coordinates: a single 12-byte load per lane.
+struct F3 {
+  float x, y, z;
+};
+
+// The query frame of a merge workgroup, staged once in LDS (raw float xyz).
+extern __shared__ F3 q_stage[];
+
+// B_LDS: the query frame b is q_stage (the pointer argument is unused).
+// The arithmetic is the same either way, so both give the same bits.
+template <bool B_LDS>
+__device__ __forceinline__ double wave_exact_score_t(const float* __restrict__ a,
+                                                     const double* __restrict__ ca, double ga,
+                                                     const float* __restrict__ b,
+                                                     const double* __restrict__ cb, double gb,
+                                                     const float* __restrict__ w, int natoms,
+                                                     bool nofit) {
   const int lane = threadIdx.x & 63;
   double S[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) S[i] = 0.0;
   const double ca0 = ca[0], ca1 = ca[1], ca2 = ca[2];
   const double cb0 = cb[0], cb1 = cb[1], cb2 = cb[2];
+  const F3* __restrict__ a3 = reinterpret_cast<const F3*>(a);
+  const F3* __restrict__ b3 = reinterpret_cast<const F3*>(b);
+#pragma unroll 2
   for (int i = lane; i < natoms; i += 64) {
     const double wi = (double)w[i];
-    const double ax = (double)a[3 * i] - ca0, ay = (double)a[3 * i + 1] - ca1,
-                 az = (double)a[3 * i + 2] - ca2;
-    const double bx = (double)b[3 * i] - cb0, by = (double)b[3 * i + 1] - cb1,
-                 bz = (double)b[3 * i + 2] - cb2;
+    const F3 av = a3[i];
+    const F3 bv = B_LDS ? q_stage[i] : b3[i];
+    const double ax = (double)av.x - ca0, ay = (double)av.y - ca1, az = (double)av.z - ca2;
+    const double bx = (double)bv.x - cb0, by = (double)bv.y - cb1, bz = (double)bv.z - cb2;
     const double wx = wi * ax, wy = wi * ay, wz = wi * az;
     S[0] += wx * bx; S[1] += wx * by; S[2] += wx * bz;
     S[3] += wy * bx; S[4] += wy * by; S[5] += wy * bz;
@@ -763,6 +777,15 @@ __device__ __forceinline__ double wave_exact_score(const float* __restrict__ a,
   for (int i = 0; i < 9; ++i) S[i] = wave_sum_d(S[i]);
   const double lam = nofit ? (S[0] + S[4] + S[8]) : qcp_lambda_exact(S, ga, gb);
   return fmax(ga + gb - 2.0 * lam, 0.0);
+}
+
+__device__ __forceinline__ double wave_exact_score(const float* __restrict__ a,
+                                                   const double* __restrict__ ca, double ga,
+                                                   const float* __restrict__ b,
+                                                   const double* __restrict__ cb, double gb,
+                                                   const float* __restrict__ w, int natoms,
+                                                   bool nofit) {
+  return wave_exact_score_t<false>(a, ca, ga, b, cb, gb, w, natoms, nofit);
 }
 
 // Exact FP64 score of one pair of data vectors (a: reference, b: query),
@@ -812,12 +835,13 @@ struct PairSource {
   const double* faux;
 };
 
+template <bool B_LDS = false>
 __device__ __forceinline__ double wave_pair_score(const PairSource& P, int64_t r, int64_t q,
                                                   bool nofit) {
   if (P.metric == kMetricRMSD)
-    return wave_exact_score(P.rxyz + r * (int64_t)P.natoms * 3, P.rcen + r * 3, P.rG[r],
-                            P.fxyz + q * (int64_t)P.natoms * 3, P.fcen + q * 3, P.fG[q], P.w,
-                            P.natoms, nofit);
+    return wave_exact_score_t<B_LDS>(P.rxyz + r * (int64_t)P.natoms * 3, P.rcen + r * 3, P.rG[r],
+                                     P.fxyz + q * (int64_t)P.natoms * 3, P.fcen + q * 3, P.fG[q],
+                                     P.w, P.natoms, nofit);
   return wave_data_score(P.r64 + r * (int64_t)P.vdim, P.raux + r * 2, P.f64 + q * (int64_t)P.vdim,
                          P.faux + q * 2, P.vdim, P.metric);
 }
@@ -827,6 +851,7 @@ __device__ __forceinline__ double wave_pair_score(const PairSource& P, int64_t r
 
 struct MergeParams {
   PairSource src;
+  const int* order;  // query rows in processing order (nullptr: 0..n_fit-1)
   double W, out_scale;
   const uint2* ent;
   const int* cnt;
@@ -861,8 +886,14 @@ __device__ void block_bitonic_sort(double* s, int* idx, int n_pow2) {
   __syncthreads();
 }
 
+// STAGE: frames whose query frame fits kMergeStageAtoms are re-scored
+// against an LDS copy of it, so each candidate reads only its reference
+// frame from memory.
+constexpr int kMergeStageAtoms = 2048;
+
+template <bool STAGE>
 __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
-  const int64_t q = blockIdx.x;
+  const int64_t q = P.order ? P.order[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   __shared__ unsigned hist[256];
   __shared__ unsigned s_prefix;
@@ -940,8 +971,13 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
     atomicAdd(P.n_cand, nc);
   }
 
+  if constexpr (STAGE) {
+    const F3* fx = reinterpret_cast<const F3*>(P.src.fxyz) + q * (int64_t)P.src.natoms;
+    for (int i = tid; i < P.src.natoms; i += 256) q_stage[i] = fx[i];
+    __syncthreads();
+  }
   for (int c = wave; c < nc; c += 4) {
-    const double s = wave_pair_score(P.src, cand_idx[c], q, P.nofit);
+    const double s = wave_pair_score<STAGE>(P.src, cand_idx[c], q, P.nofit);
     if (lane == 0) cand_s[c] = s;
   }
   int np2 = 64;
@@ -1324,7 +1360,7 @@ static PairSource pair_source(const DevFrames& ref, const DevFrames& fit, const 
 hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
                         const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
                         bool nofit, double* out_dist, int* out_idx, int* fallback, int* n_cand,
-                        hipStream_t st) {
+                        const int* order, hipStream_t st) {
   MergeParams P;
   P.src = pair_source(ref, fit, w);
   P.W = w.W;
@@ -1343,7 +1379,12 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
   P.out_idx = out_idx;
   P.fallback = fallback;
   P.n_cand = n_cand;
-  hipLaunchKernelGGL(merge_kernel, dim3((unsigned)fit.n), dim3(256), 0, st, P);
+  P.order = order;
+  if (w.metric == kMetricRMSD && ref.natoms <= kMergeStageAtoms)
+    hipLaunchKernelGGL(merge_kernel<true>, dim3((unsigned)fit.n), dim3(256),
+                       (size_t)ref.natoms * sizeof(F3), st, P);
+  else
+    hipLaunchKernelGGL(merge_kernel<false>, dim3((unsigned)fit.n), dim3(256), 0, st, P);
   return hipGetLastError();
 }
 
