@@ -124,6 +124,7 @@ def run_serving(knn, synth, oracle, prec, args):
         f"{sizes.sum()} frames ({sizes.mean():.0f} per request)")
     lat, done_t, kept = [], [], {}
     with knn.KnnContext(0, prec) as ctx:
+        ctx.set_option(knn.OPT_SERVE_GRAPHS, args.serve_graphs)
         ctx.set_reference(res, w)
         for nw in (4096, 1, 700, 4096):  # warm-up: sizes the batch slots
             ctx.wait(ctx.submit(pool[:nw], kq))
@@ -160,7 +161,8 @@ def run_serving(knn, synth, oracle, prec, args):
             outstanding.append((j, r))
         retire(block=True)
         st1 = ctx.serve_stats()
-        st = {key: st1[key] - st0[key] for key in ("batches", "frames", "busy_ms", "fallback_rows")}
+        st = {key: st1[key] - st0[key]
+              for key in ("batches", "frames", "busy_ms", "fallback_rows", "graph_launches")}
         st["max_batch_frames"], st["slots"] = st1["max_batch_frames"], st1["slots"]
         # the kept requests again, each alone: results must be bit-identical
         same = True
@@ -187,7 +189,7 @@ def run_serving(knn, synth, oracle, prec, args):
             "gpu_busy_frac": st["busy_ms"] / (span_s * 1e3),
             "batches": st["batches"], "mean_batch_frames": st["frames"] / max(1, st["batches"]),
             "max_batch_frames": st["max_batch_frames"], "slots": st["slots"],
-            "fallback_rows": st["fallback_rows"],
+            "fallback_rows": st["fallback_rows"], "graph_launches": st["graph_launches"],
             "alone_vs_mix_identical": same, "alone_vs_mix_requests": len(kept),
             "sample_max_err_nm": err,
             "mode": "host buffers in and out (PCIe included), latency from scheduled arrival"}
@@ -371,6 +373,8 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=30.0,
                     help="length of the Poisson serving run (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="mean request arrivals per second")
+    ap.add_argument("--serve-graphs", type=int, default=1,
+                    help="replay each served batch's kernels as a hipGraph (1) or launch them (0)")
     ap.add_argument("--mixed-seconds", type=float, default=0.0,
                     help="run the three-set mixed serving workload and its graph check for this "
                          "many seconds (0 = skip; the CPU side of the graph check takes minutes)")

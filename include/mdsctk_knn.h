@@ -68,6 +68,8 @@ extern "C" {
 #define MDSCTK_OPT_REF_SPLITS 3   /* force the number of reference phases (0 = automatic) */
 #define MDSCTK_OPT_SERVE_SLOTS 4  /* batches in flight at once when serving (1..4, default 2) */
 #define MDSCTK_OPT_SERVE_MAX_FRAMES 5 /* query frames per served batch (default 8192) */
+#define MDSCTK_OPT_SERVE_GRAPHS 6   /* 1 (default): each served batch's kernels are captured
+                                       into a hipGraph and replayed; 0: launched directly */
 
 typedef struct mdsctk_knn_ctx mdsctk_knn_ctx;
 
@@ -177,6 +179,8 @@ typedef struct mdsctk_knn_serve_stats {
   int32_t slots;
   int32_t slot_frames;     /* frames per batch after the HBM budget clamp */
   double busy_ms;          /* wall time with at least one batch in flight */
+  int64_t graph_launches;  /* batches replayed as a hipGraph */
+  int64_t graph_builds;    /* executable graphs instantiated (the others were updated) */
 } mdsctk_knn_serve_stats;
 
 /* Queue a query (same arguments and output layout as mdsctk_knn_query;

@@ -466,15 +466,15 @@ size_t fused_list_entries(const mdsctk_knn_ctx* c, int64_t n_fit, int k1) {
   return (size_t)n_fit * phases * 4 * list_capacity_for(k1);
 }
 
-int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
-                 FusedInfo* info) {
-  DevFrames& fit = s.fit;
+int plan_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, FusedPlan* p) {
+  const DevFrames& fit = s.fit;
   const int64_t n_fit = fit.n;
   const int cap = list_capacity_for(q.k1);
-  const int grid = contract_grid(c);
-  const int phases = fused_phases(c, fit.n_tiles);
-  Lists L;
-  L.nl = phases * 4;
+  p->grid = contract_grid(c);
+  p->phases = fused_phases(c, fit.n_tiles);
+  Lists& L = p->L;
+  L = Lists();
+  L.nl = p->phases * 4;
   L.cap = cap;
   const size_t n_hdr = (size_t)n_fit * L.nl;
   const size_t ent_bytes = n_hdr * cap * sizeof(uint2);
@@ -489,24 +489,39 @@ int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist
   L.cnt = (int*)sp; sp += n_hdr * sizeof(int);
   L.ovf = (float*)sp; sp += n_hdr * sizeof(float);
   L.thr = (unsigned*)sp; sp += n_fit * sizeof(unsigned);
-  float* eps2 = (float*)sp; sp += n_fit * sizeof(float);
-  int* d_fb = (int*)sp;
-  const double kappa = (c->w.metric == kMetricRMSD ? kappa_for(c->prec, c->ref.natoms)
-                                                   : kappa_data(c->prec, c->ref.natoms)) *
-                       c->eps_scale;
-  HIP_TRY(c, launch_init_lists(L, n_fit, s.st));
-  HIP_TRY(c, launch_eps(fit, c->ref, kappa, eps2, s.gmax, s.st));
-  HIP_TRY(c, hipMemsetAsync(s.n_cand, 0, sizeof(int), s.st));
-  HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
-  HIP_TRY(c, launch_contract(c->ref, fit, L, eps2, q.k1, phases, q.nofit, nullptr, grid, s.st));
-  HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
-  HIP_TRY(c, launch_merge(c->ref, fit, c->w, L, eps2, q.k1, q.k_out, q.rank_off, q.nofit, d_dist,
-                          d_idx, d_fb, s.n_cand, nullptr, s.st));
-  HIP_TRY(c, hipEventRecord(s.ev[4], s.st));
-  info->phases = phases;
-  info->cap = cap;
-  info->kappa = kappa;
-  info->d_fallback = d_fb;
+  p->eps2 = (float*)sp; sp += n_fit * sizeof(float);
+  p->d_fallback = (int*)sp;
+  p->kappa = (c->w.metric == kMetricRMSD ? kappa_for(c->prec, c->ref.natoms)
+                                         : kappa_data(c->prec, c->ref.natoms)) *
+             c->eps_scale;
+  return 0;
+}
+
+int enqueue_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, const FusedPlan& p,
+                  double* d_dist, int* d_idx, bool timing) {
+  const DevFrames& fit = s.fit;
+  HIP_TRY(c, launch_init_lists(p.L, fit.n, s.st));
+  HIP_TRY(c, launch_eps(fit, c->ref, p.kappa, p.eps2, s.gmax, s.n_cand, s.st));
+  if (timing) HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
+  HIP_TRY(c, launch_contract(c->ref, fit, p.L, p.eps2, q.k1, p.phases, q.nofit, nullptr, p.grid,
+                             s.st));
+  if (timing) HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
+  HIP_TRY(c, launch_merge(c->ref, fit, c->w, p.L, p.eps2, q.k1, q.k_out, q.rank_off, q.nofit,
+                          d_dist, d_idx, p.d_fallback, s.n_cand, nullptr, s.st));
+  if (timing) HIP_TRY(c, hipEventRecord(s.ev[4], s.st));
+  return 0;
+}
+
+int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
+                 FusedInfo* info) {
+  FusedPlan p;
+  int rc = plan_fused(c, s, q, &p);
+  if (rc) return rc;
+  if ((rc = enqueue_fused(c, s, q, p, d_dist, d_idx, true))) return rc;
+  info->phases = p.phases;
+  info->cap = p.L.cap;
+  info->kappa = p.kappa;
+  info->d_fallback = p.d_fallback;
   return 0;
 }
 
@@ -767,6 +782,10 @@ int mdsctk_knn_set_option(mdsctk_knn_ctx* c, int option, double value) {
       if (!(value >= 1 && value <= 4)) return fail(c, MDSCTK_ERR_ARG, "serve slots must be 1..4");
       if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");
       c->serve_slots = (int)value;
+      return 0;
+    case MDSCTK_OPT_SERVE_GRAPHS:
+      if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");
+      c->serve_graphs = value != 0;
       return 0;
     case MDSCTK_OPT_SERVE_MAX_FRAMES:
       if (!(value >= 1 && value <= (double)INT32_MAX))

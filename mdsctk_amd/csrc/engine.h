@@ -50,6 +50,18 @@ struct QueryShape {
   bool fused = false;  // contraction + merge (else exact rows)
 };
 
+// Workspace layout and launch shape of one fused batch, fixed before any
+// work is enqueued (so that the enqueue step allocates nothing and can be
+// captured into a graph).
+struct FusedPlan {
+  Lists L;
+  float* eps2 = nullptr;
+  int* d_fallback = nullptr;
+  int phases = 0;
+  int grid = 0;
+  double kappa = 0;
+};
+
 // What the fused pipeline did for a batch.
 struct FusedInfo {
   int phases = 0;
@@ -76,6 +88,7 @@ struct mdsctk_knn_ctx {
   int forced_phases = 0;
   int serve_slots = 2;
   int64_t serve_max_frames = 8192;
+  bool serve_graphs = true;  // replay each served batch's kernels as a hipGraph
 
   mdsctk::DevWeights w;  // masses, or unit masses plus the metric for data vectors
   mdsctk::DevFrames ref;
@@ -117,8 +130,14 @@ int prep_slot(mdsctk_knn_ctx* c, Slot& s);
 int contract_grid(const mdsctk_knn_ctx* c);
 int fused_phases(const mdsctk_knn_ctx* c, int fit_tiles);
 size_t fused_list_entries(const mdsctk_knn_ctx* c, int64_t n_fit, int k1);
-// Enqueue the fused pipeline (lists, contraction, merge) for the slot's
-// query frames on its stream; outputs to d_dist/d_idx.  Records ev[2..4].
+// The fused pipeline (lists, contraction, merge) for the slot's query
+// frames: plan_fused sizes the slot's workspace (it may allocate);
+// enqueue_fused only enqueues kernels on the slot's stream (with timing,
+// also events ev[2..4]), outputs to d_dist/d_idx; launch_fused does both
+// with timing.
+int plan_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, FusedPlan* p);
+int enqueue_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, const FusedPlan& p,
+                  double* d_dist, int* d_idx, bool timing);
 int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
                  FusedInfo* info);
 // Exact FP64 rows for the listed fit rows (nullptr = rows [0,nrows)) of

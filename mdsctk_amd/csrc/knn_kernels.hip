@@ -1153,9 +1153,14 @@ __global__ void max_kernel(const double* __restrict__ v, int64_t n, double* out)
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// Also clears the merge's candidate counter: a kernel store instead of a
+// memset node keeps the per-batch sequence free of memsets when it is
+// captured into a graph.
 __global__ void eps_kernel(const double* __restrict__ fGt, int64_t n_fit,
-                           const double* __restrict__ gmax, double kappa, float* eps2) {
+                           const double* __restrict__ gmax, double kappa, float* eps2,
+                           int* n_cand) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && n_cand) *n_cand = 0;
   if (i < n_fit) eps2[i] = (float)(2.0 * kappa * (fGt[i] + *gmax));
 }
 
@@ -1275,11 +1280,12 @@ hipError_t launch_max_gt(const DevFrames& f, double* out, hipStream_t st) {
 }
 
 hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, float* eps2,
-                      double* gmax, hipStream_t st) {
+                      double* gmax, int* n_cand, hipStream_t st) {
   hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, st, ref.Gt, (int64_t)ref.n_tiles * kTile,
                      gmax);
-  hipLaunchKernelGGL(eps_kernel, dim3((unsigned)((fit.n + 255) / 256)), dim3(256), 0, st, fit.Gt,
-                     fit.n, gmax, kappa, eps2);
+  const int64_t n = fit.n > 0 ? fit.n : 1;
+  hipLaunchKernelGGL(eps_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, fit.Gt,
+                     fit.n, gmax, kappa, eps2, n_cand);
   return hipGetLastError();
 }
 

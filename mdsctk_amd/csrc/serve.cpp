@@ -15,6 +15,7 @@
 // re-score of a certified candidate, sorted by (distance, index), computed by
 // the same per-pair code whatever rows share the batch.
 #include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -66,6 +67,7 @@ struct ServeSlot {
   int* h_fb = nullptr;
   size_t h_elems = 0, h_rows = 0;
   hipEvent_t done = nullptr;
+  hipGraphExec_t exec = nullptr;  // the slot's batch kernels, updated per batch
   int64_t limit = 0;  // frame limit of the batch in the slot
   // shape the workspace was last sized for
   int64_t sized_limit = 0;
@@ -106,6 +108,8 @@ class Server {
     cv_work_.notify_all();
     if (th_.joinable()) th_.join();
     for (auto& sl : slots_) {
+      if (sl.s.st) (void)hipStreamSynchronize(sl.s.st);
+      if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
       slot_release(c_, sl.s);
       if (sl.done) (void)hipEventDestroy(sl.done);
       free_host(sl);
@@ -167,6 +171,8 @@ class Server {
     *out = st_;
     out->slots = (int32_t)slots_.size();
     out->busy_ms = busy_ms_ + (busy_since_ >= 0 ? clock_ms(c_) - busy_since_ : 0.0);
+    out->graph_launches = graph_launches_.load();
+    out->graph_builds = graph_builds_.load();
   }
 
   bool idle() {
@@ -330,15 +336,21 @@ class Server {
                                     hipMemcpyHostToDevice, s.st);
       if (e != hipSuccess) return hip_fail(c_, e, "serve: query upload");
     }
-    if ((rc = prep_slot(c_, s))) return rc;
     if (q.fused) {
-      FusedInfo info;
-      if ((rc = launch_fused(c_, s, q, sl.d_dist, sl.d_idx, &info))) return rc;
-      sl.d_fb = info.d_fallback;
+      FusedPlan plan;
+      if ((rc = plan_fused(c_, s, q, &plan))) return rc;
+      if (c_->serve_graphs) {
+        if ((rc = replay_graph(sl, plan))) return rc;
+      } else {
+        if ((rc = prep_slot(c_, s))) return rc;
+        if ((rc = enqueue_fused(c_, s, q, plan, sl.d_dist, sl.d_idx, false))) return rc;
+      }
+      sl.d_fb = plan.d_fallback;
       if (hipMemcpyAsync(sl.h_fb, sl.d_fb, sl.n * sizeof(int), hipMemcpyDeviceToHost, s.st) !=
           hipSuccess)
         return fail(c_, MDSCTK_ERR_HIP, "serve: flag copy");
     } else {
+      if ((rc = prep_slot(c_, s))) return rc;
       if ((rc = exact_rows_select(c_, s, s.fit, nullptr, (int)sl.n, q, sl.d_dist, sl.d_idx)))
         return rc;
       std::memset(sl.h_fb, 0, sl.n * sizeof(int));
@@ -349,6 +361,50 @@ class Server {
             hipSuccess ||
         hipEventRecord(sl.done, s.st) != hipSuccess)
       return fail(c_, MDSCTK_ERR_HIP, "serve: output copy");
+    return 0;
+  }
+
+  // The batch's kernels (prep, list init, window, contraction, merge) are
+  // captured from the slot's stream into a graph.  The slot keeps one
+  // executable graph and updates it in place with each new capture (same
+  // topology, new sizes and launch shapes); it is rebuilt only when the
+  // update is refused.  Nothing in the sequence is a memset or an
+  // allocation: plan_fused has sized the workspace before the capture.
+  int replay_graph(ServeSlot& sl, const FusedPlan& plan) {
+    Slot& s = sl.s;
+    hipError_t e = hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(c_, e, "serve: graph capture");
+    int rc = prep_slot(c_, s);
+    if (!rc) rc = enqueue_fused(c_, s, sl.q, plan, sl.d_dist, sl.d_idx, false);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s.st, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : hip_fail(c_, e, "serve: graph capture");
+    }
+    if (sl.exec) {
+      hipGraphNode_t err_node = nullptr;
+      hipGraphExecUpdateResult res;
+      if (hipGraphExecUpdate(sl.exec, g, &err_node, &res) != hipSuccess ||
+          res != hipGraphExecUpdateSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(sl.exec);
+        sl.exec = nullptr;
+      }
+    }
+    if (!sl.exec) {
+      e = hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        sl.exec = nullptr;
+        (void)hipGraphDestroy(g);
+        return hip_fail(c_, e, "serve: graph instantiate");
+      }
+      ++graph_builds_;
+    }
+    (void)hipGraphDestroy(g);
+    e = hipGraphLaunch(sl.exec, s.st);
+    if (e != hipSuccess) return hip_fail(c_, e, "serve: graph launch");
+    ++graph_launches_;
     return 0;
   }
 
@@ -512,6 +568,7 @@ class Server {
   size_t avail_ = 0;  // HBM budget left when serving started (0 = no budget)
   mdsctk_knn_serve_stats st_ = {};
   double busy_ms_ = 0, busy_since_ = -1;
+  std::atomic<int64_t> graph_launches_{0}, graph_builds_{0};
   std::string err_;  // error text of the serving thread's last failure
   std::thread th_;
 };

@@ -32,6 +32,7 @@ OPT_HBM_BUDGET = 1
 OPT_EPS_SCALE = 2
 OPT_REF_SPLITS = 3
 OPT_SERVE_SLOTS = 4
+OPT_SERVE_GRAPHS = 6
 OPT_SERVE_MAX_FRAMES = 5
 
 ERR_ARG = -1
@@ -89,6 +90,8 @@ class ServeStats(ctypes.Structure):
         ("slots", ctypes.c_int32),
         ("slot_frames", ctypes.c_int32),
         ("busy_ms", ctypes.c_double),
+        ("graph_launches", ctypes.c_int64),
+        ("graph_builds", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -29,11 +29,13 @@ def _queries(sizes, natoms, seed):
     return out
 
 
-def test_serve_matches_sync_and_oracle():
+@pytest.mark.parametrize("graphs", [True, False])
+def test_serve_matches_sync_and_oracle(graphs):
     ref, w = _set(700, 48, 21)
     qs = _queries([1, 5, 40, 129, 300], 48, 21)
     k = 9
     with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_option(knn.OPT_SERVE_GRAPHS, int(graphs))
         ctx.set_reference(ref, w)
         sync = [ctx.query(q, k) for q in qs]
         reqs = [ctx.submit(q, k) for q in qs]
@@ -46,10 +48,33 @@ def test_serve_matches_sync_and_oracle():
         check_knn(ref, q, w, r.dist, r.idx, d_o, i_o)
     assert st["requests_done"] == len(qs)
     assert st["frames"] == sum(len(q) for q in qs)
+    assert st["graph_launches"] == (st["batches"] if graphs else 0)
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp32"])
-def test_alone_vs_mix_bit_identical(prec):
+def test_graph_replays_match_direct_launches():
+    """One executable graph per batch slot, updated and replayed for batch
+    after batch of different sizes and k, with the serving loop's own copies
+    and synchronisation between replays: every replay equals the direct
+    synchronous query of the same frames."""
+    ref, w = _set(900, 40, 24)
+    sizes = [3, 64, 1, 200, 17, 96, 5, 130, 40, 2, 77, 31]
+    qs = _queries(sizes, 40, 24)
+    ks = [7, 7, 7, 12, 7, 7, 3, 7, 7, 7, 12, 7]
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_option(knn.OPT_SERVE_SLOTS, 1)
+        ctx.set_reference(ref, w)
+        sync = [ctx.query(q, k) for q, k in zip(qs, ks)]
+        served = [ctx.wait(ctx.submit(q, k)) for q, k in zip(qs, ks)]   # one batch each
+        st = ctx.serve_stats()
+    for (d, i), r in zip(sync, served):
+        assert (r.dist == d).all() and (r.idx == i).all()
+    assert st["batches"] == len(qs) and st["graph_launches"] == len(qs)
+    # the executable was updated in place, not rebuilt per batch
+    assert st["graph_builds"] < len(qs)
+
+
+@pytest.mark.parametrize("prec,graphs", [("bf16", True), ("fp32", True), ("bf16", False)])
+def test_alone_vs_mix_bit_identical(prec, graphs):
     """The same requests alone (one at a time) and submitted together, with
     batches capped at 128 frames so that large requests are split over
     batches and small ones share them."""
@@ -59,6 +84,7 @@ def test_alone_vs_mix_bit_identical(prec):
     k = 12
     with knn.KnnContext(0, prec) as ctx:
         ctx.set_option(knn.OPT_SERVE_MAX_FRAMES, 128)
+        ctx.set_option(knn.OPT_SERVE_GRAPHS, int(graphs))
         ctx.set_reference(ref, w)
         alone = []
         for q in qs:
