@@ -386,10 +386,45 @@ struct ContractParams {
 // group's atoms, reading the wave's fragments from LDS.  For the split forms
 // the three products hi*hi + hi*lo + lo*hi accumulate into the same FP32
 // accumulators (the lo*lo term is dropped).
-template <int PREC>
+// TRACE: only the trace of the block is needed (--nofit and data vectors):
+// the diagonal products A_c * B_c accumulate into acc[0], acc[4], acc[8]
+// exactly as in the full product, a third of the MFMA work, and the other
+// accumulators stay zero.  (One accumulator for all three would triple the
+// FP32 accumulation chain, and with it the error of aligned frames.)
+template <int PREC, bool TRACE>
 __device__ __forceinline__ void mma_group(f32x16 (&acc)[9], const uint4* __restrict__ As,
                                           const uint4* __restrict__ Bs, int lane) {
-  if constexpr (PREC == kFP32) {
+  if constexpr (TRACE) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      f32x16& d = acc[4 * c];
+      if constexpr (PREC == kFP32) {
+        const uint4 a = As[c * 64 + lane], b = Bs[c * 64 + lane];
+        d = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.x), __uint_as_float(b.x), d,
+                                                 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.y), __uint_as_float(b.y), d,
+                                                 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.z), __uint_as_float(b.z), d,
+                                                 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.w), __uint_as_float(b.w), d,
+                                                 0, 0, 0);
+      } else {
+        const uint4 ah = As[(2 * c) * 64 + lane], al = As[(2 * c + 1) * 64 + lane];
+        const uint4 bh = Bs[(2 * c) * 64 + lane], bl = Bs[(2 * c + 1) * 64 + lane];
+#pragma unroll
+        for (int prod = 0; prod < 3; ++prod) {
+          const uint4 av = (prod == 2) ? al : ah;
+          const uint4 bv = (prod == 1) ? bl : bh;
+          if constexpr (PREC == kBF16)
+            d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av),
+                                                        __builtin_bit_cast(bf16x8, bv), d, 0, 0, 0);
+          else
+            d = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, av),
+                                                       __builtin_bit_cast(f16x8, bv), d, 0, 0, 0);
+        }
+      }
+    }
+  } else if constexpr (PREC == kFP32) {
     uint4 b[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) b[d] = Bs[d * 64 + lane];
@@ -572,7 +607,7 @@ __device__ __forceinline__ void stage_barrier() {
 
 // FT = fit tiles per workgroup (2 or 4); the workgroup has 2*FT waves, one
 // per (reference tile, fit tile) pair.
-template <int PREC, int PER, int FT, bool DBG>
+template <int PREC, int PER, int FT, bool DBG, bool TRACE>
 __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(ContractParams P) {
   constexpr int BPG = (PREC == kFP32) ? 3 : 6;
   constexpr int TILE_VEC = BPG * 64;
@@ -683,13 +718,13 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
       // this tile or the next) is being copied.
       for (int g = 0; g < ng3; g += 3) {
         dma_stage(t, g + 2, d2);
-        mma_group<PREC>(acc, a0, b0, lane);
+        mma_group<PREC, TRACE>(acc, a0, b0, lane);
         stage_barrier<BPG>();
         dma_stage(t, g + 3, d0);
-        if (g + 1 < ng) mma_group<PREC>(acc, a1, b1, lane);
+        if (g + 1 < ng) mma_group<PREC, TRACE>(acc, a1, b1, lane);
         stage_barrier<BPG>();
         dma_stage(t, g + 4, d1);
-        if (g + 2 < ng) mma_group<PREC>(acc, a2, b2, lane);
+        if (g + 2 < ng) mma_group<PREC, TRACE>(acc, a2, b2, lane);
         stage_barrier<BPG>();
       }
       // Publish the next tile's G~ (read by its epilogue, after at least one
@@ -1317,21 +1352,28 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   // FP32 runs two 4-wave workgroups per CU (2 x 2 tiles each); the split
   // forms one 8-wave workgroup per CU (2 x 4 tiles: a third less L2 traffic
   // per MFMA).
-#define MDSCTK_CONTRACT(PR, PER, FT, DBG)                                                     \
-  hipLaunchKernelGGL((contract_kernel<PR, PER, FT, DBG>), grid, dim3(64 * kRefTilesPerWG * FT), 0, \
-                     st, P)
-#define MDSCTK_CONTRACT_CAP(PR, FT)         \
-  do {                                      \
-    if (dbg_scores)                         \
-      MDSCTK_CONTRACT(PR, 4, FT, true);     \
-    else if (L.cap == 256)                  \
-      MDSCTK_CONTRACT(PR, 4, FT, false);    \
-    else if (L.cap == 512)                  \
-      MDSCTK_CONTRACT(PR, 8, FT, false);    \
-    else if (L.cap == 1024)                 \
-      MDSCTK_CONTRACT(PR, 16, FT, false);   \
-    else                                    \
-      return hipErrorInvalidValue;          \
+#define MDSCTK_CONTRACT(PR, PER, FT, DBG, TR)                                             \
+  hipLaunchKernelGGL((contract_kernel<PR, PER, FT, DBG, TR>), grid,                         \
+                     dim3(64 * kRefTilesPerWG * FT), 0, st, P)
+#define MDSCTK_CONTRACT_TR(PR, PER, FT, DBG)     \
+  do {                                           \
+    if (nofit)                                   \
+      MDSCTK_CONTRACT(PR, PER, FT, DBG, true);   \
+    else                                         \
+      MDSCTK_CONTRACT(PR, PER, FT, DBG, false);  \
+  } while (0)
+#define MDSCTK_CONTRACT_CAP(PR, FT)            \
+  do {                                         \
+    if (dbg_scores)                            \
+      MDSCTK_CONTRACT_TR(PR, 4, FT, true);     \
+    else if (L.cap == 256)                     \
+      MDSCTK_CONTRACT_TR(PR, 4, FT, false);    \
+    else if (L.cap == 512)                     \
+      MDSCTK_CONTRACT_TR(PR, 8, FT, false);    \
+    else if (L.cap == 1024)                    \
+      MDSCTK_CONTRACT_TR(PR, 16, FT, false);   \
+    else                                       \
+      return hipErrorInvalidValue;             \
   } while (0)
   switch (ref.prec) {
     case kFP32: MDSCTK_CONTRACT_CAP(kFP32, 2); break;
@@ -1340,6 +1382,7 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
     default: return hipErrorInvalidValue;
   }
 #undef MDSCTK_CONTRACT_CAP
+#undef MDSCTK_CONTRACT_TR
 #undef MDSCTK_CONTRACT
   return hipGetLastError();
 }
