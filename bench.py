@@ -370,6 +370,8 @@ def main():
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16", "fp32"])
     ap.add_argument("--fp32-steps", type=int, default=1, help="steps of the FP32 comparator (0 = skip)")
+    ap.add_argument("--fp16-steps", type=int, default=1,
+                    help="steps of the split-FP16 form, reported beside the headline (0 = skip)")
     ap.add_argument("--serving-seconds", type=float, default=30.0,
                     help="length of the Poisson serving run (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="mean request arrivals per second")
@@ -487,6 +489,20 @@ def main():
         out["fp32_path"] = {"value": pairs / wall32, "unit": "pairs/s", "ms_per_step": wall32 * 1e3,
                             "contract_ms": ms32, "achieved_tflops": a32, "frac_fp32_peak": a32 / 157.3}
         out["speedup_vs_fp32"] = out["value"] / out["fp32_path"]["value"]
+
+    if args.fp16_steps > 0 and prec != "fp16":
+        # the other split form the precision form allows: same MFMA rate, an
+        # 11-bit hi part, so a narrower candidate window
+        with knn.KnnContext(0, "fp16") as ctx:
+            ctx.set_reference(frames, w)
+            wall16, st16, d16, i16 = time_fused(ctx, hip, n, k, args.fp16_steps, 0)
+        ms16 = mean_stat(st16, "ms_contract")
+        a16 = 54 * natoms * pairs / (ms16 * 1e-3) / 1e12
+        out["fp16_path"] = {"value": pairs / wall16, "unit": "pairs/s", "ms_per_step": wall16 * 1e3,
+                            "contract_ms": ms16, "merge_ms": mean_stat(st16, "ms_merge"),
+                            "achieved_tflops": a16, "frac_fp16_peak": a16 / 2500.0,
+                            "candidates_per_query": mean_stat(st16, "n_candidates") / n,
+                            "outputs_equal_headline": bool((d16 == dist).all() and (i16 == idx).all())}
 
     if args.serving_seconds > 0:
         out["serving"] = run_serving(knn, synth, oracle, prec, args)
