@@ -150,6 +150,7 @@ void free_frames(mdsctk_knn_ctx* c, DevFrames& f) {
   if (f.owns_xyz) dfree(c, f.xyz);
   dfree(c, f.x64);
   dfree(c, f.aux);
+  dfree(c, f.gmax);
   dfree(c, f.cen);
   dfree(c, f.G);
   dfree(c, f.Gt);
@@ -193,6 +194,14 @@ int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st) {
     return 0;
   }
   HIP_TRY(c, launch_prep(f, c->w, st));
+  return 0;
+}
+
+// max Gt of a resident set, kept on the device for every later window.
+int resident_gmax(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st) {
+  int rc = dalloc(c, (void**)&f.gmax, sizeof(double), "max G term");
+  if (rc) return rc;
+  HIP_TRY(c, launch_max_gt(f, f.gmax, st));
   return 0;
 }
 
@@ -248,6 +257,7 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
   c->ref.owns_xyz = true;
   if ((rc = alloc_frames(c, c->ref, n_ref, natoms, c->prec))) return rc;
   if ((rc = prep_frames(c, c->ref, c->main.st))) return rc;
+  if ((rc = resident_gmax(c, c->ref, c->main.st))) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->main.st));
   c->has_ref = true;
   return 0;
@@ -326,6 +336,7 @@ int set_data_reference_impl(mdsctk_knn_ctx* c, const double* x, int64_t n_ref, i
   }
   HIP_TRY(c, launch_data_prep(f, metric, c->data_shift, st));
   if ((rc = prep_frames(c, f, st))) return rc;
+  if ((rc = resident_gmax(c, f, st))) return rc;
   HIP_TRY(c, hipStreamSynchronize(st));
   c->has_ref = true;
   return 0;
@@ -337,7 +348,6 @@ int slot_init(mdsctk_knn_ctx* c, Slot& s) {
   HIP_TRY(c, hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
   for (auto& e : s.ev) HIP_TRY(c, hipEventCreate(&e));
   int rc;
-  if ((rc = dalloc(c, (void**)&s.gmax, 256, "scratch"))) return rc;
   if ((rc = dalloc(c, (void**)&s.n_cand, 256, "scratch"))) return rc;
   return 0;
 }
@@ -347,7 +357,6 @@ void slot_release(mdsctk_knn_ctx* c, Slot& s) {
   free_frames(c, s.fit);
   dfree(c, s.ws_lists);
   dfree(c, s.ws_small);
-  dfree(c, s.gmax);
   dfree(c, s.n_cand);
   dfree(c, s.ws_out);
   dfree(c, s.ws_rows);
@@ -501,7 +510,7 @@ int enqueue_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, const FusedPl
                   double* d_dist, int* d_idx, bool timing) {
   const DevFrames& fit = s.fit;
   HIP_TRY(c, launch_init_lists(p.L, fit.n, s.st));
-  HIP_TRY(c, launch_eps(fit, c->ref, p.kappa, p.eps2, s.gmax, s.n_cand, s.st));
+  HIP_TRY(c, launch_eps(fit, c->ref, p.kappa, p.eps2, s.n_cand, s.st));
   if (timing) HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
   HIP_TRY(c, launch_contract(c->ref, fit, p.L, p.eps2, q.k1, p.phases, q.nofit, nullptr, p.grid,
                              s.st));

@@ -31,7 +31,6 @@ struct Slot {
   size_t ws_lists_bytes = 0;
   void* ws_small = nullptr;
   size_t ws_small_bytes = 0;
-  double* gmax = nullptr;
   int* n_cand = nullptr;
   void* ws_out = nullptr;
   size_t ws_out_bytes = 0;

@@ -59,6 +59,9 @@ struct DevFrames {
   int vdim = 0;
   double* x64 = nullptr;
   double* aux = nullptr;
+  // resident set only: one device double, max Gt over the set (the window's
+  // reference term), computed once when the set is made resident
+  double* gmax = nullptr;
 };
 
 struct DevWeights {
@@ -86,10 +89,9 @@ struct Lists {
 // Launchers (all asynchronous on `st`).
 hipError_t launch_prep(const DevFrames& f, const DevWeights& w, hipStream_t st);
 hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st);
-// eps2[q] = 2 kappa (Gt_fit[q] + max Gt_ref); gmax is one device double of
-// scratch; *n_cand (if given) is set to 0.
+// eps2[q] = 2 kappa (Gt_fit[q] + *ref.gmax); *n_cand (if given) is set to 0.
 hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, float* eps2,
-                      double* gmax, int* n_cand, hipStream_t st);
+                      int* n_cand, hipStream_t st);
 // Approximate contraction; when dbg_scores != nullptr it instead writes the
 // approximate score of every pair to dbg_scores[fit*n_ref+ref] (small sizes).
 hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lists& L,

@@ -1315,12 +1315,11 @@ hipError_t launch_max_gt(const DevFrames& f, double* out, hipStream_t st) {
 }
 
 hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, float* eps2,
-                      double* gmax, int* n_cand, hipStream_t st) {
-  hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, st, ref.Gt, (int64_t)ref.n_tiles * kTile,
-                     gmax);
+                      int* n_cand, hipStream_t st) {
+  if (!ref.gmax) return hipErrorInvalidValue;
   const int64_t n = fit.n > 0 ? fit.n : 1;
   hipLaunchKernelGGL(eps_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, fit.Gt,
-                     fit.n, gmax, kappa, eps2, n_cand);
+                     fit.n, ref.gmax, kappa, eps2, n_cand);
   return hipGetLastError();
 }
 
