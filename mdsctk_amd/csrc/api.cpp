@@ -18,17 +18,24 @@ thread_local std::string* tl_err_sink = nullptr;
 
 // Candidate window: |s~ - s| <= kappa (G_q + max G_ref) for the approximate
 // score s~ of each contraction form (s = G_q + G_r - 2 lambda).  kappa is an
-// envelope a*sqrt(N) + b fitted to the largest relative deviation measured
-// on the synthetic generators (random, metastable with near-duplicates, and
-// pre-aligned frames) for N = 20..2000 atoms, times a safety factor of 2.
-// FP32 accumulation contributes the sqrt(N) term (one rounding per MFMA
-// step); the split forms add a constant term from the dropped lo*lo'
-// product.  It is a measured statistical envelope, not a worst-case proof.
+// envelope fitted to the largest relative deviation measured on the
+// synthetic generators (random, metastable with near-duplicates, and
+// pre-aligned frames with 30 % near-duplicates; uniform and backbone masses;
+// fit and --nofit; cross and self queries) for N = 16..2000 atoms, times a
+// safety factor of 2.  FP32 accumulation contributes the a*sqrt(N) term (one
+// rounding per MFMA step).  The split forms add a representation term:
+//   FP16  a constant b = 1e-7 (11-bit hi part);
+//   BF16  c / sqrt(N) with the dithered split (prep_kernel's atom_scale),
+//         whose residues no longer align between frames, so the term
+//         averages over atoms; c = 3.4e-5 covers the largest deviation,
+//         8.5e-6 at N = 16 with --nofit (without dithering the same term was
+//         a constant 4e-6 from aligned frames, 2.4x wider at N = 1000).
+// It is a measured statistical envelope, not a worst-case proof.
 double kappa_for(int prec, int natoms) {
   const double rn = std::sqrt((double)std::max(natoms, 16));
   switch (prec) {
     case kFP32: return 2.0 * (5.5e-8 * rn);
-    case kBF16: return 2.0 * (3.0e-8 * rn + 4.0e-6);
+    case kBF16: return 2.0 * (3.0e-8 * rn + 3.4e-5 / rn);
     case kFP16: return 2.0 * (3.0e-8 * rn + 1.0e-7);
     default: return 0.0;
   }
@@ -155,6 +162,8 @@ void free_frames(mdsctk_knn_ctx* c, DevFrames& f) {
   dfree(c, f.G);
   dfree(c, f.Gt);
   dfree(c, f.op);
+  dfree(c, f.Gt_q);
+  dfree(c, f.op_q);
   f = DevFrames();
 }
 
@@ -178,7 +187,14 @@ int alloc_frames(mdsctk_knn_ctx* c, DevFrames& f, int64_t n, int natoms, int pre
   return 0;
 }
 
-int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st) {
+// The split-BF16 form of frames dithers its operands (prep_kernel's
+// atom_scale): its window then no longer pays for the coherent error of
+// aligned frames.
+bool dithered(const mdsctk_knn_ctx* c) { return c->prec == kBF16 && c->w.metric == kMetricRMSD; }
+
+// resident: the reference set (reference-side operand, plus the query-side
+// one for self queries when dithered); otherwise a query batch.
+int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st, bool resident) {
   if (c->prec == kExact) {
     // The exact form needs only centroids and G; the prep pass writes an
     // operand too, so give it a temporary one.
@@ -186,14 +202,31 @@ int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st) {
     int rc = dalloc(c, &tmp, operand_bytes(kFP32, f.n_tiles, f.n_groups), "prep scratch");
     if (rc) return rc;
     f.op = (uint4*)tmp;
-    hipError_t e = launch_prep(f, c->w, st);
+    hipError_t e = launch_prep(f, c->w, 0, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     f.op = nullptr;
     dfree(c, tmp);
     if (e != hipSuccess) return hip_fail(c, e, "prep");
     return 0;
   }
-  HIP_TRY(c, launch_prep(f, c->w, st));
+  if (!dithered(c)) {
+    HIP_TRY(c, launch_prep(f, c->w, 0, st));
+    return 0;
+  }
+  if (!resident) {
+    HIP_TRY(c, launch_prep(f, c->w, -1, st));
+    return 0;
+  }
+  HIP_TRY(c, launch_prep(f, c->w, +1, st));
+  const size_t rows = (size_t)f.n_tiles * kTile;
+  int rc;
+  if (!f.op_q &&
+      (rc = dalloc(c, (void**)&f.op_q, operand_bytes(f.prec, f.n_tiles, f.n_groups),
+                   "query-side operand")))
+    return rc;
+  if (!f.Gt_q && (rc = dalloc(c, (void**)&f.Gt_q, rows * sizeof(double), "query-side G~")))
+    return rc;
+  HIP_TRY(c, launch_prep(query_view(f), c->w, -1, st));
   return 0;
 }
 
@@ -256,7 +289,7 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
                             c->main.st));
   c->ref.owns_xyz = true;
   if ((rc = alloc_frames(c, c->ref, n_ref, natoms, c->prec))) return rc;
-  if ((rc = prep_frames(c, c->ref, c->main.st))) return rc;
+  if ((rc = prep_frames(c, c->ref, c->main.st, true))) return rc;
   if ((rc = resident_gmax(c, c->ref, c->main.st))) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->main.st));
   c->has_ref = true;
@@ -335,7 +368,7 @@ int set_data_reference_impl(mdsctk_knn_ctx* c, const double* x, int64_t n_ref, i
     HIP_TRY(c, launch_column_mean(f.x64, n_ref, vdim, c->data_shift, st));
   }
   HIP_TRY(c, launch_data_prep(f, metric, c->data_shift, st));
-  if ((rc = prep_frames(c, f, st))) return rc;
+  if ((rc = prep_frames(c, f, st, true))) return rc;
   if ((rc = resident_gmax(c, f, st))) return rc;
   HIP_TRY(c, hipStreamSynchronize(st));
   c->has_ref = true;
@@ -431,7 +464,7 @@ int slot_frames(mdsctk_knn_ctx* c, Slot& s, int64_t n_fit) {
   return 0;
 }
 
-int prep_slot(mdsctk_knn_ctx* c, Slot& s) { return prep_frames(c, s.fit, s.st); }
+int prep_slot(mdsctk_knn_ctx* c, Slot& s) { return prep_frames(c, s.fit, s.st, false); }
 
 int stage_input(mdsctk_knn_ctx* c, Slot& s, const void* in, bool on_device, int64_t n_fit) {
   int rc = slot_frames(c, s, n_fit);
@@ -631,7 +664,7 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
     // The fused pipeline runs on the slot's frames; a self query points the
     // slot at the resident set for its duration.
     DevFrames saved = s.fit;
-    if (!xyz) s.fit = c->ref;
+    if (!xyz) s.fit = query_view(c->ref);
     FusedInfo info;
     rc = launch_fused(c, s, q, d_dist, d_idx, &info);
     if (!xyz) s.fit = saved;
@@ -699,7 +732,8 @@ int debug_scores_impl(mdsctk_knn_ctx* c, const void* xyz_nm, bool data, int64_t 
   HIP_TRY(c, hipSetDevice(c->device));
   Slot& s = c->main;
   const bool nofit = flags & MDSCTK_KNN_NOFIT;
-  const DevFrames* fit = &c->ref;
+  const DevFrames self = query_view(c->ref);
+  const DevFrames* fit = &self;
   int rc;
   if (xyz_nm) {
     if ((rc = stage_input(c, s, xyz_nm, false, n_fit))) return rc;

@@ -62,7 +62,22 @@ struct DevFrames {
   // resident set only: one device double, max Gt over the set (the window's
   // reference term), computed once when the set is made resident
   double* gmax = nullptr;
+  // resident set of a dithered split form only: its operand and G~ in the
+  // query-side scaling, which a self query uses as its query operand
+  uint4* op_q = nullptr;
+  double* Gt_q = nullptr;
 };
+
+// The resident set seen as a query set (self query): the query-side operand
+// where the set has one.
+inline DevFrames query_view(const DevFrames& ref) {
+  DevFrames v = ref;
+  if (ref.op_q) {
+    v.op = ref.op_q;
+    v.Gt = ref.Gt_q;
+  }
+  return v;
+}
 
 struct DevWeights {
   int natoms = 0;
@@ -87,7 +102,9 @@ struct Lists {
 };
 
 // Launchers (all asynchronous on `st`).
-hipError_t launch_prep(const DevFrames& f, const DevWeights& w, hipStream_t st);
+// side 0: plain operand; +1 / -1: the dithered split's reference / query
+// side (split forms only; FP32 ignores it).
+hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side, hipStream_t st);
 hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st);
 // eps2[q] = 2 kappa (Gt_fit[q] + *ref.gmax); *n_cand (if given) is set to 0.
 hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, float* eps2,

@@ -217,12 +217,24 @@ __device__ __forceinline__ double qcp_score_approx(const float* m, double ga, do
 // prep: one 192-thread workgroup per 32-frame tile.  Thread t -> plane
 // c = t/64, lane l = t%64 of that plane's operand block.
 
+// Per-atom scale of the dithered split (a golden-ratio sequence in [1, 2)).
+// Reference operands carry sqrt(w) x * s_a and query operands sqrt(w) x / s_a,
+// so every product is unchanged, but the two sides round on different
+// grids: the hi/lo residues of the same coordinate no longer coincide, and
+// the dropped lo*lo' term of aligned or duplicate frames stops adding up
+// coherently.
+__device__ __forceinline__ double atom_scale(int a) {
+  const double t = (double)a * 0.6180339887498949;
+  return 1.0 + (t - floor(t));
+}
+
 template <int PREC>
 __global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz, int64_t n,
                                                    int natoms, int n_groups,
                                                    const float* __restrict__ w,
                                                    const float* __restrict__ sw, double Wtot,
-                                                   int center, double* __restrict__ cen,
+                                                   int center, int side,
+                                                   double* __restrict__ cen,
                                                    double* __restrict__ G,
                                                    double* __restrict__ Gt,
                                                    uint4* __restrict__ op) {
@@ -307,11 +319,15 @@ __global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz
       for (int j = 0; j < 8; ++j) {
         const int la = local_atom(j);
         const int a = g * KG + la;
-        double v = 0.0;
+        double v = 0.0, sc = 1.0;
         if (a < natoms) {
           const double d = (double)xs[row * CH + la * 3 + c] - cm;
           gex += (double)w[a] * d * d;
           v = (double)sw[a] * d;
+          if (side != 0) {
+            sc = side > 0 ? atom_scale(a) : 1.0 / atom_scale(a);
+            v *= sc;
+          }
         }
         double vh, vl;
         if constexpr (PREC == kBF16) {
@@ -325,10 +341,11 @@ __global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz
           lo[j] = f16_rne((float)(v - vh));
           vl = f16_val(lo[j]);
         }
-        // G~ of the represented value (hi+lo)^2: unbiased against the three-
-        // product cross term, whose dropped lo*lo' averages out between
-        // frames that are not atom-by-atom aligned.
-        gop += (vh + vl) * (vh + vl);
+        // G~ of the represented value (hi+lo)^2 (scale taken back out):
+        // unbiased against the three-product cross term, whose dropped
+        // lo*lo' averages out when the two sides' rounding is independent.
+        const double rep = (vh + vl) / sc;
+        gop += rep * rep;
       }
       uint4 oh, ol;
       oh.x = hi[0] | ((unsigned)hi[1] << 16); oh.y = hi[2] | ((unsigned)hi[3] << 16);
@@ -1278,21 +1295,21 @@ int list_capacity_for(int k1) {
   return 0;
 }
 
-hipError_t launch_prep(const DevFrames& f, const DevWeights& w, hipStream_t st) {
+hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side, hipStream_t st) {
   dim3 grid(f.n_tiles), block(192);
   const int center = w.metric == kMetricRMSD ? 1 : 0;
   switch (f.prec) {
     case kFP32:
       hipLaunchKernelGGL(prep_kernel<kFP32>, grid, block, 0, st, f.xyz, f.n, f.natoms,
-                         f.n_groups, w.w, w.sw, w.W, center, f.cen, f.G, f.Gt, f.op);
+                         f.n_groups, w.w, w.sw, w.W, center, side, f.cen, f.G, f.Gt, f.op);
       break;
     case kBF16:
       hipLaunchKernelGGL(prep_kernel<kBF16>, grid, block, 0, st, f.xyz, f.n, f.natoms,
-                         f.n_groups, w.w, w.sw, w.W, center, f.cen, f.G, f.Gt, f.op);
+                         f.n_groups, w.w, w.sw, w.W, center, side, f.cen, f.G, f.Gt, f.op);
       break;
     case kFP16:
       hipLaunchKernelGGL(prep_kernel<kFP16>, grid, block, 0, st, f.xyz, f.n, f.natoms,
-                         f.n_groups, w.w, w.sw, w.W, center, f.cen, f.G, f.Gt, f.op);
+                         f.n_groups, w.w, w.sw, w.W, center, side, f.cen, f.G, f.Gt, f.op);
       break;
     default:
       return hipErrorInvalidValue;

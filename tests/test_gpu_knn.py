@@ -35,7 +35,7 @@ def _aligned(n, natoms, seed, dup=0.3):
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("gen", ["random", "metastable", "aligned"])
-@pytest.mark.parametrize("natoms", [57, 304, 1000])
+@pytest.mark.parametrize("natoms", [16, 57, 304, 1000])
 @pytest.mark.parametrize("nofit", [False, True])
 def test_approx_scores_within_window(prec, gen, natoms, nofit):
     """The contraction's approximate score s~ stays inside the candidate
@@ -53,6 +53,7 @@ def test_approx_scores_within_window(prec, gen, natoms, nofit):
     with knn.KnnContext(0, prec) as ctx:
         ctx.set_reference(ref, w)
         a, e = ctx.debug_scores(fit, flags=flags)
+        a_self, e_self = ctx.debug_scores(None, flags=flags)   # the self-query operand
         ctx.query(fit, 3, flags)
         kappa = ctx.stats()["eps_rel"]
     W = w.astype(np.float64).sum()
@@ -62,8 +63,9 @@ def test_approx_scores_within_window(prec, gen, natoms, nofit):
     d_orc = oracle.full_exact(ref, fit, w, nofit=nofit) / 10.0
     assert np.abs(d_gpu - d_orc).max() < 1e-9
     rel = np.abs(a - e) / (Gf[:, None] + G.max())
-    print(prec, gen, natoms, nofit, "max rel", rel.max(), "kappa", kappa)
-    assert rel.max() < kappa
+    rel_self = np.abs(a_self - e_self) / (G[:, None] + G.max())
+    print(prec, gen, natoms, nofit, "max rel", rel.max(), "self", rel_self.max(), "kappa", kappa)
+    assert rel.max() < kappa and rel_self.max() < kappa
 
 
 @pytest.mark.parametrize("prec", PRECS)
