@@ -576,7 +576,9 @@ __device__ __attribute__((noinline)) void compact_lists(uint2* __restrict__ ent,
     if (keep > kcap) {
       const unsigned d = wave_select<PER>(key, kcap, 0);  // the kcap-th smallest, exactly
       dropped = __uint_as_float(d);
-      limit = d - 1u;                                     // keep key < d
+      // keep key < d; with d == 0 (more than kcap keys are exactly 0) keep
+      // the zeros and let the slot cap below drop all but kcap of them
+      limit = d == 0u ? 0u : d - 1u;
     }
     // Stream compaction in slot order e = lane*PER + j.
     const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -587,14 +589,16 @@ __device__ __attribute__((noinline)) void compact_lists(uint2* __restrict__ ent,
       kp[j] = key[j] <= limit && key[j] != kInfBits;
       pos += __popcll(__ballot(kp[j]) & lanes_below);
     }
+    // At most kcap entries survive (only the d == 0 case has more to drop):
+    // the list then has room for the next tile's 16 appends per lane.
     int wrote = 0;
 #pragma unroll
     for (int j = 0; j < PER; ++j)
       if (kp[j]) {
-        ent[base + pos + wrote] = make_uint2(key[j], val[j]);
+        if (pos + wrote < kcap) ent[base + pos + wrote] = make_uint2(key[j], val[j]);
         ++wrote;
       }
-    const int total = wave_sum_i(wrote);
+    const int total = min(wave_sum_i(wrote), kcap);
     if (lane == L) {
       cnt = total;
       thr = fminf(thr, cut);
@@ -768,8 +772,14 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
         const double s = qcp_score_approx(m, gr, gf, (double)thr, P.nofit);
         if (s <= (double)thr) {
           const float key = (float)fmax(s, 0.0);
-          P.ent[my_base + cnt] = make_uint2(__float_as_uint(key), (unsigned)ref_idx);
-          ++cnt;
+          // The compaction keeps cnt <= cap - 16 between tiles, so the guard
+          // never drops; if it did, ovf would send the row to the exact path.
+          if (cnt < P.cap) {
+            P.ent[my_base + cnt] = make_uint2(__float_as_uint(key), (unsigned)ref_idx);
+            ++cnt;
+          } else {
+            ovf = fminf(ovf, key);
+          }
         }
       }
       if constexpr (DBG) continue;
