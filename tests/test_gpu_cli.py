@@ -1,0 +1,84 @@
+"""The drop-in executables end to end on the trp-cage fixtures: knn_rms
+writes the reference's distances.dat / indices.dat (checked against the
+FP64 oracle on the decoded frames with the topology's masses), and the
+example pipeline knn_rms -> make_sysparse -> decomp_sparse reproduces the
+CPU pipeline's spectrum."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import oracle.graph as og
+from mdsctk_amd import build
+from tests.parity import check_knn
+from tests.test_formats import TRP_OOS, TRP_PDB, TRP_XTC, topology, xtc_read
+
+pytestmark = pytest.mark.gpu
+
+
+def run(tool, args, cwd):
+    return subprocess.run([tool] + args, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                          text=True, timeout=300)
+
+
+def read_rows(cwd, n, k, d="distances.dat", i="indices.dat"):
+    dist = np.fromfile(cwd / d, np.float64)
+    idx = np.fromfile(cwd / i, np.int32)
+    assert dist.size == n * k and idx.size == n * k
+    return dist.reshape(n, k), idx.reshape(n, k)
+
+
+@pytest.fixture(scope="module")
+def trp():
+    _, x, *_ = xtc_read(TRP_XTC)
+    _, y, *_ = xtc_read(TRP_OOS)
+    _, w = topology(TRP_PDB)
+    return x, y, w
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32", "exact"])
+def test_knn_rms_self_query_files(tmp_path, trp, prec):
+    x, _, w = trp
+    p = run(build.CLI_PATH, ["-k", "12", "-p", TRP_PDB, "-r", TRP_XTC, "--precision", prec],
+            tmp_path)
+    assert p.returncode == 0, p.stdout + p.stderr
+    d, i = read_rows(tmp_path, 1000, 12)
+    check_knn(x, x, w, d, i, *oracle.knn_exact(x, x, w, 12))
+
+
+def test_knn_rms_out_of_sample_and_nofit(tmp_path, trp):
+    x, y, w = trp
+    p = run(build.CLI_PATH, ["-k", "8", "-p", TRP_PDB, "-r", TRP_XTC, "-f", TRP_OOS, "-n", "true",
+                             "-d", "d.dat", "-i", "i.dat"], tmp_path)
+    assert p.returncode == 0, p.stdout + p.stderr
+    d, i = read_rows(tmp_path, y.shape[0], 8, "d.dat", "i.dat")
+    # rank 0 is dropped for an external -f too (knn_rms.cpp:246-255)
+    check_knn(x, y, w, d, i, *oracle.knn_exact(x, y, w, 8, nofit=True), nofit=True)
+
+
+def test_knn_rms_full_rows(tmp_path, trp):
+    x, _, w = trp
+    p = run(build.CLI_PATH, ["-s", "false", "-p", TRP_PDB, "-r", TRP_XTC], tmp_path)
+    assert p.returncode == 0, p.stdout + p.stderr
+    d, i = read_rows(tmp_path, 1000, 1000)
+    assert np.abs(d - oracle.full_exact(x, x, w)).max() < 1e-4
+    assert (i == np.arange(1000)[None, :]).all()
+
+
+def test_example_pipeline_spectrum(tmp_path, trp):
+    """knn_rms -k 20 | make_sysparse | decomp_sparse (the cluster_rms example
+    flow) against the same flow on the CPU: FP64 oracle kNN, the
+    edge-dictionary make_sysparse and ARPACK."""
+    x, _, w = trp
+    assert run(build.CLI_PATH, ["-k", "20", "-p", TRP_PDB, "-r", TRP_XTC], tmp_path).returncode == 0
+    assert run(build.MAKE_SYSPARSE_PATH, ["-k", "20"], tmp_path).returncode == 0
+    d, i = read_rows(tmp_path, 1000, 20)
+    sigma = float(np.median(d))
+    p = run(build.DECOMP_SPARSE_PATH, ["-q", repr(sigma), "-n", "8"], tmp_path)
+    assert p.returncode == 0, p.stdout
+    ev = np.loadtxt(tmp_path / "eigenvalues.dat")
+    d_o, i_o = oracle.knn_exact(x, x, w, 20)
+    n, pcol, irow, vals = og.ssm_build(d_o, i_o)
+    w_o, _ = og.spectral_decomp(n, pcol, irow, vals, float(np.median(d_o)), 8)
+    assert np.abs(ev - w_o).max() < 1e-6
