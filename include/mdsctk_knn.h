@@ -70,6 +70,10 @@ extern "C" {
 #define MDSCTK_OPT_SERVE_MAX_FRAMES 5 /* query frames per served batch (default 8192) */
 #define MDSCTK_OPT_SERVE_GRAPHS 6   /* 1 (default): each served batch's kernels are captured
                                        into a hipGraph and replayed; 0: launched directly */
+#define MDSCTK_OPT_SERVE_SHARE_CUS 7 /* 1 (default): each served batch's persistent
+                                        contraction takes 1/slots of the CUs, so batches in
+                                        flight run side by side; 0: each takes every CU and
+                                        batches follow one another */
 
 typedef struct mdsctk_knn_ctx mdsctk_knn_ctx;
 

@@ -826,6 +826,10 @@ int mdsctk_knn_set_option(mdsctk_knn_ctx* c, int option, double value) {
       if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");
       c->serve_slots = (int)value;
       return 0;
+    case MDSCTK_OPT_SERVE_SHARE_CUS:
+      if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");
+      c->serve_share_cus = value != 0;
+      return 0;
     case MDSCTK_OPT_SERVE_GRAPHS:
       if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");
       c->serve_graphs = value != 0;

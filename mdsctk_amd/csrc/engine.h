@@ -88,6 +88,7 @@ struct mdsctk_knn_ctx {
   int serve_slots = 2;
   int64_t serve_max_frames = 8192;
   bool serve_graphs = true;  // replay each served batch's kernels as a hipGraph
+  bool serve_share_cus = true;  // each batch's contraction takes 1/serve_slots of the CUs
 
   mdsctk::DevWeights w;  // masses, or unit masses plus the metric for data vectors
   mdsctk::DevFrames ref;

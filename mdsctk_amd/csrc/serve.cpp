@@ -339,6 +339,8 @@ class Server {
     if (q.fused) {
       FusedPlan plan;
       if ((rc = plan_fused(c_, s, q, &plan))) return rc;
+      if (c_->serve_share_cus && slots_.size() > 1)
+        plan.grid = std::max(8, (plan.grid / (int)slots_.size()) & ~7);
       if (c_->serve_graphs) {
         if ((rc = replay_graph(sl, plan))) return rc;
       } else {

@@ -33,6 +33,7 @@ OPT_EPS_SCALE = 2
 OPT_REF_SPLITS = 3
 OPT_SERVE_SLOTS = 4
 OPT_SERVE_GRAPHS = 6
+OPT_SERVE_SHARE_CUS = 7
 OPT_SERVE_MAX_FRAMES = 5
 
 ERR_ARG = -1

@@ -29,13 +29,14 @@ def _queries(sizes, natoms, seed):
     return out
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_serve_matches_sync_and_oracle(graphs):
+@pytest.mark.parametrize("graphs,share", [(True, False), (False, False), (True, True)])
+def test_serve_matches_sync_and_oracle(graphs, share):
     ref, w = _set(700, 48, 21)
     qs = _queries([1, 5, 40, 129, 300], 48, 21)
     k = 9
     with knn.KnnContext(0, "bf16") as ctx:
         ctx.set_option(knn.OPT_SERVE_GRAPHS, int(graphs))
+        ctx.set_option(knn.OPT_SERVE_SHARE_CUS, int(share))
         ctx.set_reference(ref, w)
         sync = [ctx.query(q, k) for q in qs]
         reqs = [ctx.submit(q, k) for q in qs]
