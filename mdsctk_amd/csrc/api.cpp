@@ -195,29 +195,15 @@ bool dithered(const mdsctk_knn_ctx* c) { return c->prec == kBF16 && c->w.metric 
 // resident: the reference set (reference-side operand, plus the query-side
 // one for self queries when dithered); otherwise a query batch.
 int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st, bool resident) {
-  if (c->prec == kExact) {
-    // The exact form needs only centroids and G; the prep pass writes an
-    // operand too, so give it a temporary one.
-    void* tmp = nullptr;
-    int rc = dalloc(c, &tmp, operand_bytes(kFP32, f.n_tiles, f.n_groups), "prep scratch");
-    if (rc) return rc;
-    f.op = (uint4*)tmp;
-    hipError_t e = launch_prep(f, c->w, 0, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    f.op = nullptr;
-    dfree(c, tmp);
-    if (e != hipSuccess) return hip_fail(c, e, "prep");
-    return 0;
-  }
-  if (!dithered(c)) {
-    HIP_TRY(c, launch_prep(f, c->w, 0, st));
+  // The exact form needs only centroids and G (f.op is not allocated).
+  if (c->prec == kExact || !dithered(c)) {
+    HIP_TRY(c, launch_prep(f, c->w, 0, nullptr, 0, st));
     return 0;
   }
   if (!resident) {
-    HIP_TRY(c, launch_prep(f, c->w, -1, st));
+    HIP_TRY(c, launch_prep(f, c->w, -1, nullptr, 0, st));
     return 0;
   }
-  HIP_TRY(c, launch_prep(f, c->w, +1, st));
   const size_t rows = (size_t)f.n_tiles * kTile;
   int rc;
   if (!f.op_q &&
@@ -226,12 +212,14 @@ int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st, bool resident) 
     return rc;
   if (!f.Gt_q && (rc = dalloc(c, (void**)&f.Gt_q, rows * sizeof(double), "query-side G~")))
     return rc;
-  HIP_TRY(c, launch_prep(query_view(f), c->w, -1, st));
+  const DevFrames q = query_view(f);
+  HIP_TRY(c, launch_prep(f, c->w, +1, &q, -1, st));
   return 0;
 }
 
 // max Gt of a resident set, kept on the device for every later window.
 int resident_gmax(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st) {
+  if (c->prec == kExact) return 0;  // no contraction, no window
   int rc = dalloc(c, (void**)&f.gmax, sizeof(double), "max G term");
   if (rc) return rc;
   HIP_TRY(c, launch_max_gt(f, f.gmax, st));

@@ -102,9 +102,14 @@ struct Lists {
 };
 
 // Launchers (all asynchronous on `st`).
+// The prep pass: the FP64 centroids, G and G~ (one wave per frame), then,
+// if f.op is set, the operand (one workgroup per tile and 4 K-groups).
 // side 0: plain operand; +1 / -1: the dithered split's reference / query
-// side (split forms only; FP32 ignores it).
-hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side, hipStream_t st);
+// side (split forms only).  second (may be null): a view of the same frames
+// whose op / Gt receive the operand and G~ of side2 as well (the resident
+// set's query-side operand).
+hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side,
+                       const DevFrames* second, int side2, hipStream_t st);
 hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st);
 // eps2[q] = 2 kappa (Gt_fit[q] + *ref.gmax); *n_cand (if given) is set to 0.
 hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, float* eps2,

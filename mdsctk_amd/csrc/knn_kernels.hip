@@ -217,6 +217,11 @@ __device__ __forceinline__ double qcp_score_approx(const float* m, double ga, do
 // prep: one 192-thread workgroup per 32-frame tile.  Thread t -> plane
 // c = t/64, lane l = t%64 of that plane's operand block.
 
+// One atom's coordinates: a single 12-byte load per lane.
+struct F3 {
+  float x, y, z;
+};
+
 // Per-atom scale of the dithered split (a golden-ratio sequence in [1, 2)).
 // Reference operands carry sqrt(w) x * s_a and query operands sqrt(w) x / s_a,
 // so every product is unchanged, but the two sides round on different
@@ -228,84 +233,140 @@ __device__ __forceinline__ double atom_scale(int a) {
   return 1.0 + (t - floor(t));
 }
 
+// The operand value of sqrt(w) (x - c) (already multiplied by the side's
+// dither scale) as the contraction form represents it: FP32, or hi + lo.
 template <int PREC>
-__global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz, int64_t n,
-                                                   int natoms, int n_groups,
-                                                   const float* __restrict__ w,
-                                                   const float* __restrict__ sw, double Wtot,
-                                                   int center, int side,
-                                                   double* __restrict__ cen,
-                                                   double* __restrict__ G,
-                                                   double* __restrict__ Gt,
-                                                   uint4* __restrict__ op) {
+__device__ __forceinline__ double represented(double v) {
+  if constexpr (PREC == kFP32) {
+    return (double)(float)v;
+  } else if constexpr (PREC == kBF16) {
+    const double vh = bf16_val(bf16_rne((float)v));
+    return vh + bf16_val(bf16_rne((float)(v - vh)));
+  } else {
+    const double vh = f16_val(f16_rne((float)v));
+    return vh + f16_val(f16_rne((float)(v - vh)));
+  }
+}
+
+__device__ __forceinline__ double side_scale(int side, int a) {
+  return side == 0 ? 1.0 : side > 0 ? atom_scale(a) : 1.0 / atom_scale(a);
+}
+
+// One wave per frame row (rows up to n_tiles*32; padded rows get zeros),
+// reading the frame as one contiguous run of 12-byte atoms:
+//   cen  FP64 mass-weighted centroid (center == 0, data vectors: 0)
+//   G    sum w |x - c|^2 in FP64
+//   Gt   G~ = sum of the squared operand values as the contraction form
+//        represents them (scale of the dither side taken back out); Gt2,
+//        when given, the same for a second side (the resident set's
+//        query-side operand).
+template <int PREC>
+__global__ __launch_bounds__(256) void frame_stats_kernel(
+    const float* __restrict__ xyz, int64_t n, int64_t rows, int natoms,
+    const float* __restrict__ w, const float* __restrict__ sw, double Wtot, int center,
+    int side, int side2, double* __restrict__ cen, double* __restrict__ G,
+    double* __restrict__ Gt, double* __restrict__ Gt2) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t f = (int64_t)blockIdx.x * 4 + wave;
+  if (f >= rows) return;
+  double c0 = 0.0, c1 = 0.0, c2 = 0.0, g = 0.0, gt = 0.0, gt2 = 0.0;
+  if (f < n) {
+    const F3* x = reinterpret_cast<const F3*>(xyz) + f * (int64_t)natoms;
+    if (center) {
+      for (int a = lane; a < natoms; a += 64) {
+        const double wa = (double)w[a];
+        const F3 v = x[a];
+        c0 += wa * (double)v.x;
+        c1 += wa * (double)v.y;
+        c2 += wa * (double)v.z;
+      }
+      c0 = wave_sum_d(c0) / Wtot;
+      c1 = wave_sum_d(c1) / Wtot;
+      c2 = wave_sum_d(c2) / Wtot;
+    }
+    for (int a = lane; a < natoms; a += 64) {
+      const F3 v = x[a];
+      const double d0 = (double)v.x - c0, d1 = (double)v.y - c1, d2 = (double)v.z - c2;
+      g += (double)w[a] * (d0 * d0 + d1 * d1 + d2 * d2);
+      if (Gt) {
+        const double s = (double)sw[a], sc = side_scale(side, a);
+        const double r0 = represented<PREC>(s * d0 * sc) / sc;
+        const double r1 = represented<PREC>(s * d1 * sc) / sc;
+        const double r2 = represented<PREC>(s * d2 * sc) / sc;
+        gt += r0 * r0 + r1 * r1 + r2 * r2;
+        if (Gt2) {
+          const double sc2 = side_scale(side2, a);
+          const double q0 = represented<PREC>(s * d0 * sc2) / sc2;
+          const double q1 = represented<PREC>(s * d1 * sc2) / sc2;
+          const double q2 = represented<PREC>(s * d2 * sc2) / sc2;
+          gt2 += q0 * q0 + q1 * q1 + q2 * q2;
+        }
+      }
+    }
+    g = wave_sum_d(g);
+    gt = wave_sum_d(gt);
+    gt2 = wave_sum_d(gt2);
+  }
+  if (lane == 0) {
+    cen[f * 3 + 0] = c0;
+    cen[f * 3 + 1] = c1;
+    cen[f * 3 + 2] = c2;
+    G[f] = g;
+    if (Gt) Gt[f] = gt;
+    if (Gt2) Gt2[f] = gt2;
+  }
+}
+
+// The contraction operand in fragment order, from frame_stats_kernel's
+// centroids: one workgroup per (32-frame tile, kGroupsPerStage K-groups),
+// staged through LDS once.  Thread t works on plane c = t/64, lane
+// l = t%64 of that plane's operand block.  The arithmetic of each value is
+// represented<PREC>'s.
+constexpr int kGroupsPerStage = 4;
+
+template <int PREC>
+__global__ __launch_bounds__(192) void operand_kernel(const float* __restrict__ xyz, int64_t n,
+                                                      int natoms, int n_groups,
+                                                      const float* __restrict__ sw,
+                                                      const double* __restrict__ cen, int side,
+                                                      uint4* __restrict__ op) {
   constexpr int KG = (PREC == kFP32) ? 8 : 16;
-  constexpr int NE = (PREC == kFP32) ? 4 : 8;
-  constexpr int CH = KG * 3;
-  __shared__ float xs[kTile * CH];
-  __shared__ double red[192];
-  __shared__ double scen[kTile * 3];
+  constexpr int CH = KG * 3;                     // floats of one frame in one K-group
+  constexpr int SW = CH * kGroupsPerStage;       // floats of one frame in one stage
+  constexpr int SP = SW + 1;                     // LDS row pitch: odd, so the 32 frames
+                                                 // of a plane's read fall in 32 banks
+  __shared__ float xs[kTile * SP];
 
   const int tile = blockIdx.x;
+  const int g0 = blockIdx.y * kGroupsPerStage;
   const int t = threadIdx.x;
   const int c = t >> 6;
   const int lane = t & 63;
   const int row = lane & 31;
   const int h = lane >> 5;
+  const double cm = cen[((int64_t)tile * kTile + row) * 3 + c];
 
-  auto stage = [&](int g) {
-    for (int i = t; i < kTile * CH; i += 192) {
-      const int fr = i / CH, e = i - fr * CH;
-      const int64_t ff = (int64_t)tile * kTile + fr;
-      const int a = g * KG + e / 3;
-      float v = 0.f;
-      if (ff < n && a < natoms) v = xyz[(ff * natoms + (int64_t)g * KG) * 3 + e];
-      xs[i] = v;
-    }
-  };
+  for (int i = t; i < kTile * SW; i += 192) {
+    const int fr = i / SW, e = i - fr * SW;
+    const int64_t ff = (int64_t)tile * kTile + fr;
+    const int a = g0 * KG + e / 3;
+    float v = 0.f;
+    if (ff < n && a < natoms) v = xyz[(ff * natoms + (int64_t)g0 * KG) * 3 + e];
+    xs[fr * SP + e] = v;
+  }
+  __syncthreads();
   auto local_atom = [&](int j) { return (PREC == kFP32) ? (2 * j + h) : (8 * h + j); };
 
-  // first pass: the mass-weighted centroid (skipped for data vectors, which
-  // are used as given)
-  double acc = 0.0;
-  for (int g = 0; center && g < n_groups; ++g) {
-    __syncthreads();
-    stage(g);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < NE; ++j) {
-      const int la = local_atom(j);
-      const int a = g * KG + la;
-      if (a < natoms) acc += (double)w[a] * (double)xs[row * CH + la * 3 + c];
-    }
-  }
-  red[t] = acc;
-  __syncthreads();
-  if (t < 96) {
-    const int cc = t >> 5, rr = t & 31;
-    scen[rr * 3 + cc] = center ? (red[cc * 64 + rr] + red[cc * 64 + 32 + rr]) / Wtot : 0.0;
-  }
-  __syncthreads();
-  const double cm = scen[row * 3 + c];
-
-  double gex = 0.0, gop = 0.0;
-  for (int g = 0; g < n_groups; ++g) {
-    __syncthreads();
-    stage(g);
-    __syncthreads();
+  for (int gg = 0; gg < kGroupsPerStage && g0 + gg < n_groups; ++gg) {
+    const int g = g0 + gg;
+    const float* xr = xs + row * SP + gg * CH;
     if constexpr (PREC == kFP32) {
       float v4[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int la = local_atom(j);
         const int a = g * KG + la;
-        float v = 0.f;
-        if (a < natoms) {
-          const double d = (double)xs[row * CH + la * 3 + c] - cm;
-          gex += (double)w[a] * d * d;
-          v = (float)((double)sw[a] * d);
-        }
-        v4[j] = v;
-        gop += (double)v * (double)v;
+        v4[j] = a < natoms ? (float)((double)sw[a] * ((double)xr[la * 3 + c] - cm)) : 0.f;
       }
       uint4 o;
       o.x = __float_as_uint(v4[0]);
@@ -319,33 +380,15 @@ __global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz
       for (int j = 0; j < 8; ++j) {
         const int la = local_atom(j);
         const int a = g * KG + la;
-        double v = 0.0, sc = 1.0;
-        if (a < natoms) {
-          const double d = (double)xs[row * CH + la * 3 + c] - cm;
-          gex += (double)w[a] * d * d;
-          v = (double)sw[a] * d;
-          if (side != 0) {
-            sc = side > 0 ? atom_scale(a) : 1.0 / atom_scale(a);
-            v *= sc;
-          }
-        }
-        double vh, vl;
+        double v = 0.0;
+        if (a < natoms) v = (double)sw[a] * ((double)xr[la * 3 + c] - cm) * side_scale(side, a);
         if constexpr (PREC == kBF16) {
           hi[j] = bf16_rne((float)v);
-          vh = bf16_val(hi[j]);
-          lo[j] = bf16_rne((float)(v - vh));
-          vl = bf16_val(lo[j]);
+          lo[j] = bf16_rne((float)(v - bf16_val(hi[j])));
         } else {
           hi[j] = f16_rne((float)v);
-          vh = f16_val(hi[j]);
-          lo[j] = f16_rne((float)(v - vh));
-          vl = f16_val(lo[j]);
+          lo[j] = f16_rne((float)(v - f16_val(hi[j])));
         }
-        // G~ of the represented value (hi+lo)^2 (scale taken back out):
-        // unbiased against the three-product cross term, whose dropped
-        // lo*lo' averages out when the two sides' rounding is independent.
-        const double rep = (vh + vl) / sc;
-        gop += rep * rep;
       }
       uint4 oh, ol;
       oh.x = hi[0] | ((unsigned)hi[1] << 16); oh.y = hi[2] | ((unsigned)hi[3] << 16);
@@ -356,26 +399,6 @@ __global__ __launch_bounds__(192) void prep_kernel(const float* __restrict__ xyz
       op[blk * 64 + lane] = oh;
       op[(blk + 1) * 64 + lane] = ol;
     }
-  }
-  __syncthreads();
-  red[t] = gex;
-  __syncthreads();
-  if (t < 32) {
-    double s = 0.0;
-    for (int cc = 0; cc < 3; ++cc) s += red[cc * 64 + t] + red[cc * 64 + 32 + t];
-    G[(int64_t)tile * kTile + t] = s;
-  }
-  __syncthreads();
-  red[t] = gop;
-  __syncthreads();
-  if (t < 32) {
-    double s = 0.0;
-    for (int cc = 0; cc < 3; ++cc) s += red[cc * 64 + t] + red[cc * 64 + 32 + t];
-    const int64_t f = (int64_t)tile * kTile + t;
-    Gt[f] = s;
-    cen[f * 3 + 0] = scen[t * 3 + 0];
-    cen[f * 3 + 1] = scen[t * 3 + 1];
-    cen[f * 3 + 2] = scen[t * 3 + 2];
   }
 }
 
@@ -797,11 +820,6 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
 
 // ---------------------------------------------------------------------------
 // exact FP64 score of one pair, cooperatively by a wave (lanes over atoms).
-
-// One atom's coordinates: a single 12-byte load per lane.
-struct F3 {
-  float x, y, z;
-};
 
 // The query frame of a merge workgroup, staged once in LDS (raw float xyz).
 extern __shared__ F3 q_stage[];
@@ -1305,25 +1323,32 @@ int list_capacity_for(int k1) {
   return 0;
 }
 
-hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side, hipStream_t st) {
-  dim3 grid(f.n_tiles), block(192);
+hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side,
+                       const DevFrames* second, int side2, hipStream_t st) {
+  const int64_t rows = (int64_t)f.n_tiles * kTile;
   const int center = w.metric == kMetricRMSD ? 1 : 0;
+  double* gt = f.op ? f.Gt : nullptr;
+  double* gt2 = (f.op && second) ? second->Gt : nullptr;
+  const dim3 sgrid((unsigned)((rows + 3) / 4)), sblock(256);
+  const dim3 ogrid((unsigned)f.n_tiles, (unsigned)((f.n_groups + kGroupsPerStage - 1) / kGroupsPerStage));
+#define MDSCTK_PREP(PR)                                                                       \
+  do {                                                                                        \
+    hipLaunchKernelGGL(frame_stats_kernel<PR>, sgrid, sblock, 0, st, f.xyz, f.n, rows,        \
+                       f.natoms, w.w, w.sw, w.W, center, side, side2, f.cen, f.G, gt, gt2);   \
+    if (f.op)                                                                                 \
+      hipLaunchKernelGGL(operand_kernel<PR>, ogrid, dim3(192), 0, st, f.xyz, f.n, f.natoms,   \
+                         f.n_groups, w.sw, f.cen, side, f.op);                               \
+    if (f.op && second)                                                                       \
+      hipLaunchKernelGGL(operand_kernel<PR>, ogrid, dim3(192), 0, st, f.xyz, f.n, f.natoms,   \
+                         f.n_groups, w.sw, f.cen, side2, second->op);                        \
+  } while (0)
   switch (f.prec) {
-    case kFP32:
-      hipLaunchKernelGGL(prep_kernel<kFP32>, grid, block, 0, st, f.xyz, f.n, f.natoms,
-                         f.n_groups, w.w, w.sw, w.W, center, side, f.cen, f.G, f.Gt, f.op);
-      break;
-    case kBF16:
-      hipLaunchKernelGGL(prep_kernel<kBF16>, grid, block, 0, st, f.xyz, f.n, f.natoms,
-                         f.n_groups, w.w, w.sw, w.W, center, side, f.cen, f.G, f.Gt, f.op);
-      break;
-    case kFP16:
-      hipLaunchKernelGGL(prep_kernel<kFP16>, grid, block, 0, st, f.xyz, f.n, f.natoms,
-                         f.n_groups, w.w, w.sw, w.W, center, side, f.cen, f.G, f.Gt, f.op);
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case kFP32: MDSCTK_PREP(kFP32); break;
+    case kBF16: MDSCTK_PREP(kBF16); break;
+    case kFP16: MDSCTK_PREP(kFP16); break;
+    default: return hipErrorInvalidValue;
   }
+#undef MDSCTK_PREP
   return hipGetLastError();
 }
 
