@@ -25,7 +25,7 @@ thread_local std::string* tl_err_sink = nullptr;
 // safety factor of 2.  FP32 accumulation contributes the a*sqrt(N) term (one
 // rounding per MFMA step).  The split forms add a representation term:
 //   FP16  a constant b = 1e-7 (11-bit hi part);
-//   BF16  c / sqrt(N) with the dithered split (prep_kernel's atom_scale),
+//   BF16  c / sqrt(N) with the dithered split (operand_kernel's atom_scale),
 //         whose residues no longer align between frames, so the term
 //         averages over atoms; c = 3.4e-5 covers the largest deviation,
 //         8.5e-6 at N = 16 with --nofit (without dithering the same term was
@@ -187,7 +187,7 @@ int alloc_frames(mdsctk_knn_ctx* c, DevFrames& f, int64_t n, int natoms, int pre
   return 0;
 }
 
-// The split-BF16 form of frames dithers its operands (prep_kernel's
+// The split-BF16 form of frames dithers its operands (operand_kernel's
 // atom_scale): its window then no longer pays for the coherent error of
 // aligned frames.
 bool dithered(const mdsctk_knn_ctx* c) { return c->prec == kBF16 && c->w.metric == kMetricRMSD; }

@@ -52,7 +52,7 @@ struct DevFrames {
   double* cen = nullptr;  // [n_tiles*32][3] FP64 mass-weighted centroid
   double* G = nullptr;    // [n_tiles*32] sum w |x-c|^2, FP64
   double* Gt = nullptr;   // [n_tiles*32] same sum over the rounded operand
-  uint4* op = nullptr;    // contraction operand, fragment-major (see prep kernel)
+  uint4* op = nullptr;    // contraction operand, fragment-major (see operand_kernel)
   bool owns_xyz = true;
   // data vectors only: the FP64 vectors [n][vdim] the exact scores use and,
   // for the correlation metric, [n][2] (mean, sample standard deviation)

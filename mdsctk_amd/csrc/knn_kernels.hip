@@ -1,16 +1,19 @@
 // gfx950 kernels of the knn_rms engine.
 //
 // Pipeline for one query batch (reference: knn_rms.cpp:231-291):
-//   prep_kernel      one pass per 32-frame tile: FP64 mass-weighted centroid,
-//                    G = sum w|x-c|^2 (FP64), and the contraction operand
+//   frame_stats_kernel
+//                    one wave per frame: FP64 mass-weighted centroid,
+//                    G = sum w|x-c|^2 (FP64) and G~ of the operand.
+//   operand_kernel   per (32-frame tile, 4 K-groups): the contraction operand
 //                    sqrt(w)(x-c) rounded to FP32 or split into hi/lo 16-bit
-//                    halves, written fragment-major (below).  HBM-bound.
+//                    halves (dithered for BF16), written fragment-major
+//                    (below).  HBM-bound.
 //   contract_kernel  all-pairs 3x3 cross-covariance blocks as one dense MFMA
 //                    contraction (9 accumulator tiles per wave: planes c,d of
 //                    32 ref x 32 fit frames), FP64 QCP Newton epilogue giving
 //                    an approximate score s~ = G_r + G_f - 2 lambda_max, and
-//                    per-lane candidate lists (threshold + append, wave
-//                    bitonic compaction).  No distance matrix is written.
+//                    per-lane candidate lists (threshold + append, ballot
+//                    select compaction).  No distance matrix is written.
 //   merge_kernel     per query: radix-select the (k+1)-th smallest s~ over
 //                    its lists, take every candidate within the certified
 //                    window, re-score them exactly in FP64 from the raw
