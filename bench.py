@@ -79,12 +79,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_fused(ctx, hip, n_fit, k, steps, warmup):
+def time_fused(ctx, hip, n_fit, k, steps, warmup, first=None):
+    """Self query of the resident set: `warmup` untimed steps, then `steps`
+    timed ones.  first (a dict) receives the first call's wall time, which
+    includes the engine's one-time workspace allocation."""
     d_dist = hip.malloc(n_fit * k * 8)
     d_idx = hip.malloc(n_fit * k * 4)
     try:
-        for _ in range(warmup):
+        for w in range(warmup):
+            t0 = time.perf_counter()
             ctx.query_device(0, n_fit, k, 0, d_dist, d_idx)
+            hip.sync()
+            if w == 0 and first is not None:
+                first["ms"] = (time.perf_counter() - t0) * 1e3
         hip.sync()
         stats = []
         t0 = time.perf_counter()
@@ -321,7 +328,8 @@ def measure_traffic(args):
     work = tempfile.mkdtemp(prefix="mdsctk_pmc_", dir="/tmp")
     probe = ["--copy-probe", str(COPY_PROBE_BYTES)]
     child = ["--traffic-child", "--n-frames", str(args.n_frames), "--natoms", str(args.natoms),
-             "--k", str(args.k), "--precision", args.precision, "--seed", str(args.seed)]
+             "--k", str(args.k), "--precision", args.precision, "--seed", str(args.seed),
+             "--self-symmetric", str(args.self_symmetric)]
     r = {}
     for tag, counter, a in (("copy_fetch", "FETCH_SIZE", probe), ("copy_write", "WRITE_SIZE", probe),
                             ("fetch", "FETCH_SIZE", child), ("write", "WRITE_SIZE", child)):
@@ -353,6 +361,7 @@ def traffic_child(args):
     frames = synth.metastable_trajectory(args.n_frames, args.natoms, args.seed)
     w = synth.masses(args.natoms, "uniform")
     with knn.KnnContext(0, args.precision) as ctx:
+        ctx.set_option(knn.OPT_SELF_SYMMETRIC, args.self_symmetric)
         ctx.set_reference(frames, w)
         time_fused(ctx, hip, args.n_frames, args.k, 1, 0)
     print(json.dumps({"traffic_child": True}), flush=True)
@@ -372,6 +381,12 @@ def main():
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16", "fp32"])
     ap.add_argument("--fp32-steps", type=int, default=1, help="steps of the FP32 comparator (0 = skip)")
+    ap.add_argument("--self-symmetric", type=int, default=1,
+                    help="score each unordered pair once through the n x n score matrix (1) or "
+                         "every pair through the candidate lists (0)")
+    ap.add_argument("--list-steps", type=int, default=1,
+                    help="steps of the candidate-list path, reported beside a symmetric headline "
+                         "(0 = skip)")
     ap.add_argument("--fp16-steps", type=int, default=1,
                     help="steps of the split-FP16 form, reported beside the headline (0 = skip)")
     ap.add_argument("--serving-seconds", type=float, default=30.0,
@@ -444,15 +459,24 @@ def main():
                      "epilogue": "FP64 G terms + FP64 QCP Newton, FP64 re-score of the candidate window"}
 
     pairs = float(n) * n
+    first = {}
     with knn.KnnContext(0, prec) as ctx:
+        ctx.set_option(knn.OPT_SELF_SYMMETRIC, args.self_symmetric)
         t = time.time()
         ctx.set_reference(frames, w)
         log(f"reference resident in HBM ({time.time() - t:.1f} s)")
-        wall, stats, dist, idx = time_fused(ctx, hip, n, k, args.steps, args.warmup)
+        wall, stats, dist, idx = time_fused(ctx, hip, n, k, args.steps, args.warmup, first)
     ms_contract = mean_stat(stats, "ms_contract")
+    computed = mean_stat(stats, "n_pairs_computed")
     out["value"] = pairs / wall
     out["ms_per_step"] = wall * 1e3
-    flops = FLOPS_PER_ATOM[prec] * natoms * pairs
+    out["config"]["self_symmetric"] = bool(computed < pairs)
+    out["config"]["pairs_per_step"] = pairs
+    out["config"]["pairs_scored_per_step"] = computed
+    if "ms" in first:
+        out["first_query_ms"] = first["ms"]
+    # algorithmic MFMA work of the pairs the contraction actually scored
+    flops = FLOPS_PER_ATOM[prec] * natoms * computed
     achieved = flops / (ms_contract * 1e-3) / 1e12
     out["roofline"] = {"bound": "mfma", "kernel": "contract_kernel",
                        "achieved": achieved, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
@@ -485,12 +509,26 @@ def main():
     out["accuracy"] = {"max_err_nm": err_nm, "tolerance_nm": 1e-5, "pairs_checked": 10000,
                        "rows_checked": int(len(chk)), "index_set_match": match}
 
+    if args.list_steps > 0 and computed < pairs:
+        # the same query through the candidate-list path (every pair scored)
+        with knn.KnnContext(0, prec) as ctx:
+            ctx.set_option(knn.OPT_SELF_SYMMETRIC, 0)
+            ctx.set_reference(frames, w)
+            wall_l, st_l, d_l, i_l = time_fused(ctx, hip, n, k, args.list_steps, 1)
+        ms_l = mean_stat(st_l, "ms_contract")
+        a_l = FLOPS_PER_ATOM[prec] * natoms * pairs / (ms_l * 1e-3) / 1e12
+        out["list_path"] = {"value": pairs / wall_l, "unit": "pairs/s", "ms_per_step": wall_l * 1e3,
+                            "contract_ms": ms_l, "merge_ms": mean_stat(st_l, "ms_merge"),
+                            "achieved_tflops": a_l, "frac_peak": a_l / PEAK_TFLOPS[prec],
+                            "outputs_equal_headline": bool((d_l == dist).all() and (i_l == idx).all())}
+
     if args.fp32_steps > 0 and prec != "fp32":
         with knn.KnnContext(0, "fp32") as ctx:
+            ctx.set_option(knn.OPT_SELF_SYMMETRIC, args.self_symmetric)
             ctx.set_reference(frames, w)
-            wall32, st32, _, _ = time_fused(ctx, hip, n, k, args.fp32_steps, 0)
+            wall32, st32, _, _ = time_fused(ctx, hip, n, k, args.fp32_steps, 1)
         ms32 = mean_stat(st32, "ms_contract")
-        a32 = 18 * natoms * pairs / (ms32 * 1e-3) / 1e12
+        a32 = 18 * natoms * mean_stat(st32, "n_pairs_computed") / (ms32 * 1e-3) / 1e12
         out["fp32_path"] = {"value": pairs / wall32, "unit": "pairs/s", "ms_per_step": wall32 * 1e3,
                             "contract_ms": ms32, "achieved_tflops": a32, "frac_fp32_peak": a32 / 157.3}
         out["speedup_vs_fp32"] = out["value"] / out["fp32_path"]["value"]
@@ -499,10 +537,11 @@ def main():
         # the other split form the precision form allows: same MFMA rate, an
         # 11-bit hi part, so a narrower candidate window
         with knn.KnnContext(0, "fp16") as ctx:
+            ctx.set_option(knn.OPT_SELF_SYMMETRIC, args.self_symmetric)
             ctx.set_reference(frames, w)
-            wall16, st16, d16, i16 = time_fused(ctx, hip, n, k, args.fp16_steps, 0)
+            wall16, st16, d16, i16 = time_fused(ctx, hip, n, k, args.fp16_steps, 1)
         ms16 = mean_stat(st16, "ms_contract")
-        a16 = 54 * natoms * pairs / (ms16 * 1e-3) / 1e12
+        a16 = 54 * natoms * mean_stat(st16, "n_pairs_computed") / (ms16 * 1e-3) / 1e12
         out["fp16_path"] = {"value": pairs / wall16, "unit": "pairs/s", "ms_per_step": wall16 * 1e3,
                             "contract_ms": ms16, "merge_ms": mean_stat(st16, "ms_merge"),
                             "achieved_tflops": a16, "frac_fp16_peak": a16 / 2500.0,

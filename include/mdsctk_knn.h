@@ -74,6 +74,15 @@ extern "C" {
                                         contraction takes 1/slots of the CUs, so batches in
                                         flight run side by side; 0: each takes every CU and
                                         batches follow one another */
+#define MDSCTK_OPT_SELF_SYMMETRIC 8 /* 1 (default): a self query (mdsctk_knn_query with
+                                       xyz NULL) scores each unordered pair once: the
+                                       contraction covers the upper block triangle and
+                                       writes an n x n FP32 score matrix (4 n^2 bytes, plus
+                                       16 KiB per frame of candidate lists), from which each
+                                       row's candidates are selected.  Used only when that
+                                       fits the free device memory and the HBM budget;
+                                       outputs are identical either way.  0: the candidate-
+                                       list path */
 
 typedef struct mdsctk_knn_ctx mdsctk_knn_ctx;
 
@@ -91,6 +100,8 @@ typedef struct mdsctk_knn_stats {
   int32_t ref_splits;      /* reference phases used by the contraction */
   int32_t list_capacity;   /* per-list candidate capacity */
   double eps_rel;          /* candidate window relative to (G_q + max G_ref) */
+  int64_t n_pairs_computed; /* pairs the contraction scored: n_pairs, or about half of
+                               them for a symmetric self query */
 } mdsctk_knn_stats;
 
 /* Create a context on HIP device `device` with the given MDSCTK_PREC_*.

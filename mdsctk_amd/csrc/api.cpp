@@ -381,6 +381,7 @@ void slot_release(mdsctk_knn_ctx* c, Slot& s) {
   dfree(c, s.n_cand);
   dfree(c, s.ws_out);
   dfree(c, s.ws_rows);
+  dfree(c, s.ws_sym);
   for (auto& e : s.ev)
     if (e) (void)hipEventDestroy(e);
   if (s.st) (void)hipStreamDestroy(s.st);
@@ -552,6 +553,79 @@ int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist
   info->cap = p.L.cap;
   info->kappa = p.kappa;
   info->d_fallback = p.d_fallback;
+  info->pairs_computed = s.fit.n * c->ref.n;
+  return 0;
+}
+
+int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
+                     FusedInfo* info) {
+  const DevFrames& fit = s.fit;
+  const int64_t n = fit.n;
+  if (n != c->ref.n || fit.n_tiles != c->ref.n_tiles || n < 1) return 1;
+  const int64_t ld = (n + 3) & ~int64_t(3);
+  const size_t sym_bytes = (size_t)n * (size_t)ld * sizeof(float);
+  const int cap = kSymListCap;
+  const size_t ent_bytes = (size_t)n * cap * sizeof(uint2);
+  // Room: the free device memory plus what the two buffers already hold,
+  // less a reserve for everything else; and the context's budget.
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(c, hipMemGetInfo(&free_b, &total_b));
+  const size_t have = s.ws_sym_bytes + s.ws_lists_bytes;
+  const size_t reserve = size_t(2) << 30;
+  if (sym_bytes + ent_bytes + reserve > free_b + have) return 1;
+  {
+    std::lock_guard<std::mutex> g(c->mem_mu);
+    if (c->budget && c->held - std::min(c->held, have) + sym_bytes + ent_bytes > c->budget)
+      return 1;
+  }
+  const size_t small = (size_t)n * (sizeof(int) + sizeof(float)) +
+                       (size_t)n * (sizeof(float) + sizeof(int)) + 256;
+  int rc = ensure(c, &s.ws_sym, &s.ws_sym_bytes, sym_bytes, "score matrix");
+  if (!rc) rc = ensure(c, &s.ws_lists, &s.ws_lists_bytes, ent_bytes, "candidate lists");
+  if (!rc) rc = ensure(c, &s.ws_small, &s.ws_small_bytes, small, "list headers");
+  if (rc == MDSCTK_ERR_NOMEM) {  // the list path instead, without the matrix
+    dfree(c, s.ws_sym);
+    s.ws_sym = nullptr;
+    s.ws_sym_bytes = 0;
+    c->err.clear();
+    return 1;
+  }
+  if (rc) return rc;
+  Lists L;
+  L.nl = 1;
+  L.cap = cap;
+  L.ent = (uint2*)s.ws_lists;
+  char* sp = (char*)s.ws_small;
+  double* gmax_q = (double*)sp; sp += 256;
+  L.cnt = (int*)sp; sp += n * sizeof(int);
+  L.ovf = (float*)sp; sp += n * sizeof(float);
+  float* eps2 = (float*)sp; sp += n * sizeof(float);
+  int* d_fallback = (int*)sp;
+  const double kappa = (c->w.metric == kMetricRMSD ? kappa_for(c->prec, c->ref.natoms)
+                                                   : kappa_data(c->prec, c->ref.natoms)) *
+                       c->eps_scale;
+  const int grid = contract_grid(c);
+  const int phases = fused_phases(c, fit.n_tiles);
+  float* sym = (float*)s.ws_sym;
+  HIP_TRY(c, launch_eps_sym(fit, c->ref, kappa, gmax_q, eps2, s.n_cand, s.st));
+  HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
+  HIP_TRY(c, launch_contract(c->ref, fit, L, eps2, q.k1, phases, q.nofit, nullptr, grid, s.st,
+                             sym, ld));
+  HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
+  HIP_TRY(c, launch_sym_select(sym, ld, n, eps2, q.k1, L, s.st));
+  HIP_TRY(c, launch_merge(c->ref, fit, c->w, L, eps2, q.k1, q.k_out, q.rank_off, q.nofit, d_dist,
+                          d_idx, d_fallback, s.n_cand, nullptr, s.st));
+  HIP_TRY(c, hipEventRecord(s.ev[4], s.st));
+  // pairs of the upper block triangle: fit block fb against frames from
+  // its own first frame on
+  const int64_t blk = (int64_t)kTile * contract_fit_tiles(c->prec);
+  int64_t pairs = 0;
+  for (int64_t f0 = 0; f0 < n; f0 += blk) pairs += std::min(blk, n - f0) * (n - f0);
+  info->phases = phases;
+  info->cap = cap;
+  info->kappa = kappa;
+  info->d_fallback = d_fallback;
+  info->pairs_computed = pairs;
   return 0;
 }
 
@@ -611,6 +685,7 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
   Slot& s = c->main;
   mdsctk_knn_stats S = {};
   S.n_pairs = n_fit * c->ref.n;
+  S.n_pairs_computed = S.n_pairs;
   HIP_TRY(c, hipEventRecord(s.ev[0], s.st));
 
   // Stage (or alias) the query frames and run their prep pass.
@@ -654,7 +729,9 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
     DevFrames saved = s.fit;
     if (!xyz) s.fit = query_view(c->ref);
     FusedInfo info;
-    rc = launch_fused(c, s, q, d_dist, d_idx, &info);
+    rc = 1;
+    if (!xyz && c->self_sym) rc = launch_fused_sym(c, s, q, d_dist, d_idx, &info);
+    if (rc == 1) rc = launch_fused(c, s, q, d_dist, d_idx, &info);
     if (!xyz) s.fit = saved;
     if (rc) return rc;
     std::vector<int> fb(n_fit);
@@ -669,6 +746,7 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
     S.ref_splits = info.phases;
     S.list_capacity = info.cap;
     S.eps_rel = info.kappa;
+    S.n_pairs_computed = info.pairs_computed;
     std::vector<int> rows;
     for (int64_t i = 0; i < n_fit; ++i)
       if (fb[i]) rows.push_back((int)i);
@@ -817,6 +895,9 @@ int mdsctk_knn_set_option(mdsctk_knn_ctx* c, int option, double value) {
     case MDSCTK_OPT_SERVE_SHARE_CUS:
       if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");
       c->serve_share_cus = value != 0;
+      return 0;
+    case MDSCTK_OPT_SELF_SYMMETRIC:
+      c->self_sym = value != 0;
       return 0;
     case MDSCTK_OPT_SERVE_GRAPHS:
       if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");

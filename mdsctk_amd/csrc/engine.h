@@ -36,6 +36,8 @@ struct Slot {
   size_t ws_out_bytes = 0;
   void* ws_rows = nullptr;
   size_t ws_rows_bytes = 0;
+  void* ws_sym = nullptr;  // score matrix of a symmetric self query
+  size_t ws_sym_bytes = 0;
   hipEvent_t ev[6] = {};
 };
 
@@ -67,6 +69,7 @@ struct FusedInfo {
   int cap = 0;
   double kappa = 0;
   int* d_fallback = nullptr;  // [n_fit] device flags (in the slot's workspace)
+  int64_t pairs_computed = 0;  // pairs the contraction scored
 };
 
 class Server;  // serve.cpp
@@ -89,6 +92,7 @@ struct mdsctk_knn_ctx {
   int64_t serve_max_frames = 8192;
   bool serve_graphs = true;  // replay each served batch's kernels as a hipGraph
   bool serve_share_cus = true;  // each batch's contraction takes 1/serve_slots of the CUs
+  bool self_sym = true;  // self queries through the score matrix when it fits
 
   mdsctk::DevWeights w;  // masses, or unit masses plus the metric for data vectors
   mdsctk::DevFrames ref;
@@ -140,6 +144,14 @@ int enqueue_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, const FusedPl
                   double* d_dist, int* d_idx, bool timing);
 int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
                  FusedInfo* info);
+// A self query (the slot's frames are the resident set's query view)
+// through the n x n FP32 score matrix: the contraction scores the upper block
+// triangle only, each row's candidates are selected from the matrix, and
+// the merge re-scores them as for the lists, with timing.  Returns 1 with
+// nothing enqueued when the matrix and its lists do not fit the free device
+// memory (or the context's budget).
+int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
+                     FusedInfo* info);
 // Exact FP64 rows for the listed fit rows (nullptr = rows [0,nrows)) of
 // the slot's query frames, synchronously on the slot's stream.
 int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const int* h_rows,

@@ -15,6 +15,9 @@ constexpr int kTile = 32;  // frames per MFMA tile edge (v_mfma_*_32x32xK)
 constexpr int kRefTilesPerWG = 2;
 // Frame sets are padded to a multiple of this many tiles.
 constexpr int kTileAlign = 4;
+// List capacity of a self query through the score matrix (one list per
+// row): the merge re-scores at most this many candidates per row.
+constexpr int kSymListCap = 2048;
 
 enum Prec : int { kFP32 = 0, kBF16 = 1, kFP16 = 2, kExact = 3 };
 
@@ -114,11 +117,25 @@ hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st);
 // eps2[q] = 2 kappa (Gt_fit[q] + *ref.gmax); *n_cand (if given) is set to 0.
 hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, float* eps2,
                       int* n_cand, hipStream_t st);
-// Approximate contraction; when dbg_scores != nullptr it instead writes the
-// approximate score of every pair to dbg_scores[fit*n_ref+ref] (small sizes).
+// Approximate contraction into candidate lists.  When dbg_scores != nullptr
+// it instead writes the approximate score of every pair to
+// dbg_scores[fit*n_ref+ref] (small sizes).  When sym != nullptr (a self
+// query: fit is the reference set's query view) it contracts the upper block
+// triangle only and writes every pair's FP32 score to sym[fit*sym_ld+ref]
+// (sym_ld >= n, a multiple of 4; 16-byte aligned).
 hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lists& L,
                            const float* eps2, int k1, int ref_phases, bool nofit,
-                           double* dbg_scores, int grid_wgs, hipStream_t st);
+                           double* dbg_scores, int grid_wgs, hipStream_t st,
+                           float* sym = nullptr, int64_t sym_ld = 0);
+// The window of a self query through the score matrix (fit: the reference
+// set's query view): eps2[r] = 2 kappa max(G~q[r] + max G~ref,
+// G~ref[r] + max G~q); *gmax_q receives max G~q.
+hipError_t launch_eps_sym(const DevFrames& fit, const DevFrames& ref, double kappa,
+                          double* gmax_q, float* eps2, int* n_cand, hipStream_t st);
+// Candidate lists (L.nl == 1) from the n x n score matrix of a self query:
+// per row, every key up to the window of its k1-th smallest key.
+hipError_t launch_sym_select(const float* sym, int64_t sym_ld, int64_t n, const float* eps2,
+                             int k1, const Lists& L, hipStream_t st);
 // order: the query rows in the order the merge workgroups take them
 // (nullptr: 0..n_fit-1).
 hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,

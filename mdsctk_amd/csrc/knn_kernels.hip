@@ -423,7 +423,19 @@ struct ContractParams {
   int nl, cap;
   int nofit;
   double* dbg;
+  float* sym;      // kOutSym: the score matrix, row stride sym_ld
+  int64_t sym_ld;
 };
+
+// What the contraction's epilogue produces.
+constexpr int kOutLists = 0;  // per-query candidate lists (the fused pipeline)
+constexpr int kOutDebug = 1;  // every score in FP64 (dbg, n_fit x n_ref)
+// A self query (the query set is the reference set, tile for tile): only
+// the upper block triangle is contracted -- reference pairs that start at
+// or after the workgroup's fit block -- and each score is stored in FP32 at
+// sym[f][r] and, off the diagonal blocks, at sym[r][f] as well, so every
+// entry of the n x n matrix is written exactly once.
+constexpr int kOutSym = 2;
 
 // One K-group of MFMA work for a wave: acc[c*3+d] += A_c * B_d^T over the
 // group's atoms, reading the wave's fragments from LDS.  For the split forms
@@ -653,8 +665,9 @@ __device__ __forceinline__ void stage_barrier() {
 }
 
 // FT = fit tiles per workgroup (2 or 4); the workgroup has 2*FT waves, one
-// per (reference tile, fit tile) pair.
-template <int PREC, int PER, int FT, bool DBG, bool TRACE>
+// per (reference tile, fit tile) pair.  MODE: kOutLists, kOutDebug or
+// kOutSym.
+template <int PREC, int PER, int FT, int MODE, bool TRACE>
 __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(ContractParams P) {
   constexpr int BPG = (PREC == kFP32) ? 3 : 6;
   constexpr int TILE_VEC = BPG * 64;
@@ -709,7 +722,14 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
     const int fb = item / P.phases;
     const int ph = item - fb * P.phases;
     const int n_tiles = (P.ref_pairs - ph + P.phases - 1) / P.phases;  // ref pairs ph, ph+R, ...
-    if (n_tiles <= 0) continue;
+    // First tile of the item: 0, or for kOutSym the first reference pair at
+    // or after the fit block (pair p holds frames 64p.., the block 32*FT*fb..).
+    int t0 = 0;
+    if constexpr (MODE == kOutSym) {
+      const int p_min = fb * FT / kRefTilesPerWG;
+      if (p_min > ph) t0 = (p_min - ph + P.phases - 1) / P.phases;
+    }
+    if (t0 >= n_tiles) continue;
     const int64_t fit_idx = (int64_t)(FT * fb + fi) * kTile + (lane & 31);
     const bool fvalid = fit_idx < P.n_fit;
     const int list_id = (ph * 2 + ri) * 2 + (lane >> 5);
@@ -743,12 +763,13 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
     };
 
     __syncthreads();  // previous item's readers of sbuf*/sGr are done
-    if (tid < 2 * kTile) sGr[0][tid] = P.rGt[(int64_t)ph * 2 * kTile + tid];
-    dma_stage(0, 0, d0);
-    dma_stage(0, 1, d1);
+    if (tid < 2 * kTile)
+      sGr[t0 & 1][tid] = P.rGt[(int64_t)(ph + t0 * P.phases) * 2 * kTile + tid];
+    dma_stage(t0, 0, d0);
+    dma_stage(t0, 1, d1);
     stage_barrier<BPG>();
 
-    for (int t = 0; t < n_tiles; ++t) {
+    for (int t = t0; t < n_tiles; ++t) {
       const int p = ph + t * P.phases;
       // Loads consumed only at the end of the tile: the next tile's G~ and
       // the shared per-query threshold.
@@ -786,6 +807,49 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
       // Epilogue: lane holds fit frame (lane%32) against 16 ref frames.
       const double* const gr_tile = sGr[t & 1];
       const int64_t ref_base = (int64_t)(kRefTilesPerWG * p + ri) * kTile;
+      if constexpr (MODE == kOutSym) {
+        // Scores r = 4j..4j+3 of a lane are ref frames 8j+4h .. 8j+4h+3
+        // (h = lane/32): one 16-byte store per j in the query's row; the
+        // mirrored stores of a half-wave fill 32 consecutive entries of a
+        // reference frame's row.
+        const bool mirror = p >= (fb + 1) * FT / kRefTilesPerWG;
+        float key[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int rl = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          float m[9];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) m[q] = acc[q][r];
+          // sign bit cleared: the select pass orders keys by their bits
+          key[r] = __uint_as_float(
+              __float_as_uint((float)fmax(
+                  qcp_score_approx(m, gr_tile[ri * kTile + rl], gf, INFINITY, P.nofit), 0.0)) &
+              0x7fffffffu);
+        }
+        if (fvalid) {
+          float* const row = P.sym + fit_idx * P.sym_ld;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int64_t c0 = ref_base + 8 * j + 4 * (lane >> 5);
+            if (c0 + 3 < P.n_ref) {
+              *reinterpret_cast<float4*>(row + c0) =
+                  make_float4(key[4 * j], key[4 * j + 1], key[4 * j + 2], key[4 * j + 3]);
+            } else {
+#pragma unroll
+              for (int e = 0; e < 4; ++e)
+                if (c0 + e < P.n_ref) row[c0 + e] = key[4 * j + e];
+            }
+          }
+          if (mirror) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              const int64_t ref_idx = ref_base + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+              if (ref_idx < P.n_ref) P.sym[ref_idx * P.sym_ld + fit_idx] = key[r];
+            }
+          }
+        }
+        continue;
+      }
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int rl = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -795,7 +859,7 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
 #pragma unroll
         for (int q = 0; q < 9; ++q) m[q] = acc[q][r];
         const double gr = gr_tile[ri * kTile + rl];
-        if constexpr (DBG) {
+        if constexpr (MODE == kOutDebug) {
           P.dbg[fit_idx * P.n_ref + ref_idx] = qcp_score_approx(m, gr, gf, INFINITY, P.nofit);
           continue;
         }
@@ -812,13 +876,13 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
           }
         }
       }
-      if constexpr (DBG) continue;
+      if constexpr (MODE == kOutDebug) continue;
       const float thr_before = thr;
       compact_lists<PER>(P.ent, my_base, cnt, thr, ovf, e2, P.k1, P.cap, fvalid);
       thr = fminf(thr, __shfl_xor(thr, 32));
       if (fvalid && P.thr && thr < thr_before) atomicMin(&P.thr[fit_idx], __float_as_uint(thr));
     }
-    if (fvalid && !DBG) {
+    if (fvalid && MODE == kOutLists) {
       P.cnt[hdr] = cnt;
       P.ovf[hdr] = ovf;
     }
@@ -1082,6 +1146,121 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// Self query through the score matrix: each query row's candidates, in the
+// candidate-list form the merge reads (one list per row).
+
+constexpr int kSymBins = 4096;
+
+// Calls f(index, key bits) for every key of a row: 16-byte loads over the
+// aligned part, then the tail.
+template <typename F>
+__device__ __forceinline__ void sym_row_visit(const float* __restrict__ row, int64_t n, F&& f) {
+  const int64_t n4 = n >> 2;
+  const float4* __restrict__ r4 = reinterpret_cast<const float4*>(row);
+  for (int64_t i = threadIdx.x; i < n4; i += blockDim.x) {
+    const float4 v = r4[i];
+    f(4 * i, __float_as_uint(v.x));
+    f(4 * i + 1, __float_as_uint(v.y));
+    f(4 * i + 2, __float_as_uint(v.z));
+    f(4 * i + 3, __float_as_uint(v.w));
+  }
+  for (int64_t i = 4 * n4 + threadIdx.x; i < n; i += blockDim.x) f(i, __float_as_uint(row[i]));
+}
+
+// The bin of a 4096-bin histogram that holds rank *kk (0-based); *kk
+// becomes the rank within that bin.  256 threads, 16 bins each.
+__device__ int sym_find_bin(const unsigned* hist, unsigned* scan, unsigned& kk) {
+  __shared__ int s_bin;
+  __shared__ unsigned s_rank;
+  const int tid = threadIdx.x;
+  unsigned local = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) local += hist[tid * 16 + j];
+  scan[tid] = local;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {  // inclusive Hillis-Steele scan
+    const unsigned v = tid >= off ? scan[tid - off] : 0u;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  const unsigned incl = scan[tid], excl = incl - local;
+  if (excl <= kk && kk < incl) {
+    unsigned cum = excl;
+    int b = tid * 16;
+    while (cum + hist[b] <= kk) cum += hist[b++];
+    s_bin = b;
+    s_rank = kk - cum;
+  }
+  __syncthreads();
+  kk = s_rank;
+  return s_bin;
+}
+
+// One workgroup per query row q of the n x n FP32 score matrix.  Keys are
+// non-negative floats, so their bits order like the values.  Two histogram
+// passes (bits 30..19, then 18..7 within the first bin) bound the row's
+// k1-th smallest key tau from above by the upper edge U of a bin 2^7 ulps
+// wide.  The merge's window, tau (1 + 2.4e-7) + eps2, is monotone in tau, so
+// every key up to U (1 + 2.4e-7) + eps2 goes into the row's list; the merge
+// then finds tau and the window exactly among them.  Past `cap` entries the
+// smallest dropped key goes to ovf, which makes the merge take the exact
+// path if it falls inside the window.
+__global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict__ S, int64_t ld,
+                                                         int64_t n, const float* __restrict__ eps2,
+                                                         int k1, uint2* __restrict__ ent,
+                                                         int* __restrict__ cnt,
+                                                         float* __restrict__ ovf, int cap) {
+  __shared__ unsigned hist[kSymBins];
+  __shared__ unsigned scan[256];
+  __shared__ int s_n;
+  __shared__ unsigned s_ovf;
+  const int64_t q = blockIdx.x;
+  const float* const row = S + q * ld;
+  const int tid = threadIdx.x;
+
+  unsigned kk = (unsigned)(k1 - 1);
+  for (int i = tid; i < kSymBins; i += 256) hist[i] = 0u;
+  __syncthreads();
+  sym_row_visit(row, n,
+                [&](int64_t, unsigned u) { atomicAdd(&hist[(u >> 19) & (kSymBins - 1)], 1u); });
+  __syncthreads();
+  const unsigned b1 = (unsigned)sym_find_bin(hist, scan, kk);
+  for (int i = tid; i < kSymBins; i += 256) hist[i] = 0u;
+  __syncthreads();
+  sym_row_visit(row, n, [&](int64_t, unsigned u) {
+    if ((u >> 19) == b1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
+  });
+  __syncthreads();
+  const unsigned b2 = (unsigned)sym_find_bin(hist, scan, kk);
+
+  // U = the first float above the fine bin (b1 < 0xff0: keys are finite)
+  const unsigned ub = (((b1 << 12) | b2) + 1u) << 7;
+  const float cut = b1 >= 0xff0u ? INFINITY : __uint_as_float(ub) * (1.0f + 2.4e-7f) + eps2[q];
+  const unsigned cut_bits = __float_as_uint(cut);
+  if (tid == 0) {
+    s_n = 0;
+    s_ovf = kInfBits;
+  }
+  __syncthreads();
+  uint2* const out = ent + q * (int64_t)cap;
+  sym_row_visit(row, n, [&](int64_t i, unsigned u) {
+    if (u <= cut_bits) {
+      const int pos = atomicAdd(&s_n, 1);
+      if (pos < cap)
+        out[pos] = make_uint2(u, (unsigned)i);
+      else
+        atomicMin(&s_ovf, u);
+    }
+  });
+  __syncthreads();
+  if (tid == 0) {
+    cnt[q] = min(s_n, cap);
+    ovf[q] = __uint_as_float(s_ovf);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // exact rows
 
 struct ExactParams {
@@ -1251,6 +1430,20 @@ __global__ void eps_kernel(const double* __restrict__ fGt, int64_t n_fit,
   if (i < n_fit) eps2[i] = (float)(2.0 * kappa * (fGt[i] + *gmax));
 }
 
+// Self query through the score matrix: row r also receives the scores of
+// pairs computed the other way round (r's reference-side operand against
+// f's query-side one), whose error bound is kappa (G~ref[r] + G~q[f]); the
+// window covers both kinds.
+__global__ void eps_sym_kernel(const double* __restrict__ qGt, const double* __restrict__ rGt,
+                               int64_t n, const double* __restrict__ gmax_ref,
+                               const double* __restrict__ gmax_q, double kappa, float* eps2,
+                               int* n_cand) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && n_cand) *n_cand = 0;
+  if (i < n)
+    eps2[i] = (float)(2.0 * kappa * fmax(qGt[i] + *gmax_ref, rGt[i] + *gmax_q));
+}
+
 __global__ void identity_kernel(int64_t n_fit, int64_t n_ref, int* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_fit * n_ref) out[i] = (int)(i % n_ref);
@@ -1382,9 +1575,21 @@ hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, 
   return hipGetLastError();
 }
 
+hipError_t launch_eps_sym(const DevFrames& fit, const DevFrames& ref, double kappa,
+                          double* gmax_q, float* eps2, int* n_cand, hipStream_t st) {
+  if (!ref.gmax || fit.n != ref.n) return hipErrorInvalidValue;
+  hipError_t e = launch_max_gt(fit, gmax_q, st);
+  if (e != hipSuccess) return e;
+  const int64_t n = fit.n > 0 ? fit.n : 1;
+  hipLaunchKernelGGL(eps_sym_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, fit.Gt,
+                     ref.Gt, fit.n, ref.gmax, gmax_q, kappa, eps2, n_cand);
+  return hipGetLastError();
+}
+
 hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lists& L,
                            const float* eps2, int k1, int ref_phases, bool nofit,
-                           double* dbg_scores, int grid_wgs, hipStream_t st) {
+                           double* dbg_scores, int grid_wgs, hipStream_t st, float* sym,
+                           int64_t sym_ld) {
   ContractParams P;
   P.rop = ref.op;
   P.fop = fit.op;
@@ -1406,32 +1611,38 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   P.cap = L.cap;
   P.nofit = nofit ? 1 : 0;
   P.dbg = dbg_scores;
+  P.sym = sym;
+  P.sym_ld = sym_ld;
+  if (sym && (fit.n != ref.n || fit.n_tiles != ref.n_tiles || sym_ld < ref.n || (sym_ld & 3)))
+    return hipErrorInvalidValue;
   dim3 grid(grid_wgs);
   // FP32 runs two 4-wave workgroups per CU (2 x 2 tiles each); the split
   // forms one 8-wave workgroup per CU (2 x 4 tiles: a third less L2 traffic
   // per MFMA).
-#define MDSCTK_CONTRACT(PR, PER, FT, DBG, TR)                                             \
-  hipLaunchKernelGGL((contract_kernel<PR, PER, FT, DBG, TR>), grid,                         \
+#define MDSCTK_CONTRACT(PR, PER, FT, MODE, TR)                                            \
+  hipLaunchKernelGGL((contract_kernel<PR, PER, FT, MODE, TR>), grid,                        \
                      dim3(64 * kRefTilesPerWG * FT), 0, st, P)
-#define MDSCTK_CONTRACT_TR(PR, PER, FT, DBG)     \
-  do {                                           \
-    if (nofit)                                   \
-      MDSCTK_CONTRACT(PR, PER, FT, DBG, true);   \
-    else                                         \
-      MDSCTK_CONTRACT(PR, PER, FT, DBG, false);  \
+#define MDSCTK_CONTRACT_TR(PR, PER, FT, MODE)     \
+  do {                                            \
+    if (nofit)                                    \
+      MDSCTK_CONTRACT(PR, PER, FT, MODE, true);   \
+    else                                          \
+      MDSCTK_CONTRACT(PR, PER, FT, MODE, false);  \
   } while (0)
-#define MDSCTK_CONTRACT_CAP(PR, FT)            \
-  do {                                         \
-    if (dbg_scores)                            \
-      MDSCTK_CONTRACT_TR(PR, 4, FT, true);     \
-    else if (L.cap == 256)                     \
-      MDSCTK_CONTRACT_TR(PR, 4, FT, false);    \
-    else if (L.cap == 512)                     \
-      MDSCTK_CONTRACT_TR(PR, 8, FT, false);    \
-    else if (L.cap == 1024)                    \
-      MDSCTK_CONTRACT_TR(PR, 16, FT, false);   \
-    else                                       \
-      return hipErrorInvalidValue;             \
+#define MDSCTK_CONTRACT_CAP(PR, FT)                 \
+  do {                                              \
+    if (dbg_scores)                                 \
+      MDSCTK_CONTRACT_TR(PR, 4, FT, kOutDebug);     \
+    else if (sym)                                   \
+      MDSCTK_CONTRACT_TR(PR, 4, FT, kOutSym);       \
+    else if (L.cap == 256)                          \
+      MDSCTK_CONTRACT_TR(PR, 4, FT, kOutLists);     \
+    else if (L.cap == 512)                          \
+      MDSCTK_CONTRACT_TR(PR, 8, FT, kOutLists);     \
+    else if (L.cap == 1024)                         \
+      MDSCTK_CONTRACT_TR(PR, 16, FT, kOutLists);    \
+    else                                            \
+      return hipErrorInvalidValue;                  \
   } while (0)
   switch (ref.prec) {
     case kFP32: MDSCTK_CONTRACT_CAP(kFP32, 2); break;
@@ -1492,6 +1703,15 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
                        (size_t)ref.natoms * sizeof(F3), st, P);
   else
     hipLaunchKernelGGL(merge_kernel<false>, dim3((unsigned)fit.n), dim3(256), 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_sym_select(const float* sym, int64_t sym_ld, int64_t n, const float* eps2,
+                             int k1, const Lists& L, hipStream_t st) {
+  if (L.nl != 1 || k1 < 1 || k1 > n || sym_ld < n || (sym_ld & 3)) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(sym_select_kernel, dim3((unsigned)n), dim3(256), 0, st, sym, sym_ld, n, eps2,
+                     k1, L.ent, L.cnt, L.ovf, L.cap);
   return hipGetLastError();
 }
 

@@ -34,6 +34,7 @@ OPT_REF_SPLITS = 3
 OPT_SERVE_SLOTS = 4
 OPT_SERVE_GRAPHS = 6
 OPT_SERVE_SHARE_CUS = 7
+OPT_SELF_SYMMETRIC = 8
 OPT_SERVE_MAX_FRAMES = 5
 
 ERR_ARG = -1
@@ -59,6 +60,7 @@ class Stats(ctypes.Structure):
         ("ref_splits", ctypes.c_int32),
         ("list_capacity", ctypes.c_int32),
         ("eps_rel", ctypes.c_double),
+        ("n_pairs_computed", ctypes.c_int64),
     ]
 
     def as_dict(self):
