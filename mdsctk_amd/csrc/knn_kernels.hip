@@ -1150,21 +1150,36 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
 // candidate-list form the merge reads (one list per row).
 
 constexpr int kSymBins = 4096;
+constexpr int kSymStage = 4096;  // keys staged in LDS by the second pass
 
-// Calls f(index, key bits) for every key of a row: 16-byte loads over the
-// aligned part, then the tail.
+// Calls f(index, key bits) for every key of a row (256 threads): 16-byte
+// loads, four in flight per thread, over the aligned part, then the tail.
 template <typename F>
 __device__ __forceinline__ void sym_row_visit(const float* __restrict__ row, int64_t n, F&& f) {
   const int64_t n4 = n >> 2;
   const float4* __restrict__ r4 = reinterpret_cast<const float4*>(row);
-  for (int64_t i = threadIdx.x; i < n4; i += blockDim.x) {
+  int64_t i = threadIdx.x;
+  for (; i + 3 * 256 < n4; i += 4 * 256) {
+    float4 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = r4[i + j * 256];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t e = 4 * (i + j * 256);
+      f(e, __float_as_uint(v[j].x));
+      f(e + 1, __float_as_uint(v[j].y));
+      f(e + 2, __float_as_uint(v[j].z));
+      f(e + 3, __float_as_uint(v[j].w));
+    }
+  }
+  for (; i < n4; i += 256) {
     const float4 v = r4[i];
     f(4 * i, __float_as_uint(v.x));
     f(4 * i + 1, __float_as_uint(v.y));
     f(4 * i + 2, __float_as_uint(v.z));
     f(4 * i + 3, __float_as_uint(v.w));
   }
-  for (int64_t i = 4 * n4 + threadIdx.x; i < n; i += blockDim.x) f(i, __float_as_uint(row[i]));
+  for (int64_t e = 4 * n4 + threadIdx.x; e < n; e += 256) f(e, __float_as_uint(row[e]));
 }
 
 // The bin of a 4096-bin histogram that holds rank *kk (0-based); *kk
@@ -1197,15 +1212,45 @@ __device__ int sym_find_bin(const unsigned* hist, unsigned* scan, unsigned& kk) 
   return s_bin;
 }
 
-// One workgroup per query row q of the n x n FP32 score matrix.  Keys are
-// non-negative floats, so their bits order like the values.  Two histogram
-// passes (bits 30..19, then 18..7 within the first bin) bound the row's
-// k1-th smallest key tau from above by the upper edge U of a bin 2^7 ulps
-// wide.  The merge's window, tau (1 + 2.4e-7) + eps2, is monotone in tau, so
-// every key up to U (1 + 2.4e-7) + eps2 goes into the row's list; the merge
-// then finds tau and the window exactly among them.  Past `cap` entries the
-// smallest dropped key goes to ovf, which makes the merge take the exact
-// path if it falls inside the window.
+// Upper edge (the first float above) of fine bin b2 inside coarse bin b1.
+__device__ __forceinline__ unsigned sym_bin_top(unsigned b1, unsigned b2) {
+  return (((b1 << 12) | b2) + 1u) << 7;
+}
+
+// The window bound of a key bound: top (1 + 2.4e-7) + eps2, the merge's
+// window formula (monotone in its argument).  Keys are finite, so a top
+// at or beyond the infinity bits only arises from a bound that covers all.
+__device__ __forceinline__ unsigned sym_window_bits(unsigned top, float e2) {
+  const float w = top >= kInfBits ? INFINITY : __uint_as_float(top) * (1.0f + 2.4e-7f) + e2;
+  return __float_as_uint(w);
+}
+
+constexpr int64_t kSymSampleMin = 32768;  // rows at least this long sample first
+
+// One 256-thread workgroup per query row q of the n x n FP32 score matrix:
+// the row's candidate list for the merge.  Keys are non-negative floats, so
+// their bits order like the values; tau is the row's k1-th smallest key, and
+// the merge's window is tau (1 + 2.4e-7) + eps2, monotone in tau.  A bin
+// below is 2^7 ulps wide (key bits 30..19 coarse, 18..7 fine).
+//
+// Sampled form (rows of kSymSampleMin keys or more): a sample of 1/16 of
+// the row (16 evenly spaced runs) gives T, the top of the fine bin of its
+// rank-r key, r about twice k1/16.  One pass over the row stages every key
+// up to the window of T in LDS and counts the keys below T.  If at least k1
+// are below T, then tau < T, every key up to the window of tau is staged,
+// and tau's bin and window are found among the staged keys alone.
+//
+// Full form (short rows, or the sampled form's bound failed or staged too
+// many keys): a histogram of the whole row finds tau's coarse bin b1; a
+// second pass lists the keys below b1 (all <= tau) and stages those from b1
+// up to the window of b1's top; the fine bin comes from the staged keys
+// (or, past kSymStage of them, from a third pass over the row, which then
+// also gathers).
+//
+// Every key up to the window of tau's bin top joins the list, so the list
+// holds the merge's whole window.  Past `cap` entries the smallest dropped
+// key goes to ovf, which makes the merge take the exact path if it falls
+// inside the window.
 __global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict__ S, int64_t ld,
                                                          int64_t n, const float* __restrict__ eps2,
                                                          int k1, uint2* __restrict__ ent,
@@ -1213,46 +1258,138 @@ __global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict
                                                          float* __restrict__ ovf, int cap) {
   __shared__ unsigned hist[kSymBins];
   __shared__ unsigned scan[256];
-  __shared__ int s_n;
+  __shared__ uint2 stage[kSymStage];
+  __shared__ int s_n, s_ns, s_nt;
   __shared__ unsigned s_ovf;
   const int64_t q = blockIdx.x;
   const float* const row = S + q * ld;
   const int tid = threadIdx.x;
+  const float e2 = eps2[q];
+  uint2* const out = ent + q * (int64_t)cap;
+  auto put = [&](unsigned u, unsigned i) {
+    const int pos = atomicAdd(&s_n, 1);
+    if (pos < cap)
+      out[pos] = make_uint2(u, i);
+    else
+      atomicMin(&s_ovf, u);
+  };
+  auto clear_hist = [&]() {
+    for (int i = tid; i < kSymBins; i += 256) hist[i] = 0u;
+  };
+  // tau's fine bin among the staged keys [0, ns) (all of the row's keys up
+  // to the staged range's top must be there); then the list from them
+  auto select_staged = [&](int ns, unsigned kk) {
+    clear_hist();
+    __syncthreads();
+    for (int p = tid; p < ns; p += 256) atomicAdd(&hist[(stage[p].x >> 19) & (kSymBins - 1)], 1u);
+    __syncthreads();
+    const unsigned b1 = (unsigned)sym_find_bin(hist, scan, kk);
+    clear_hist();
+    __syncthreads();
+    for (int p = tid; p < ns; p += 256) {
+      const unsigned u = stage[p].x;
+      if ((u >> 19) == b1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
+    }
+    __syncthreads();
+    const unsigned b2 = (unsigned)sym_find_bin(hist, scan, kk);
+    const unsigned cut_bits = sym_window_bits(sym_bin_top(b1, b2), e2);
+    for (int p = tid; p < ns; p += 256) {
+      const uint2 e = stage[p];
+      if (e.x <= cut_bits) put(e.x, e.y);
+    }
+  };
 
-  unsigned kk = (unsigned)(k1 - 1);
-  for (int i = tid; i < kSymBins; i += 256) hist[i] = 0u;
-  __syncthreads();
-  sym_row_visit(row, n,
-                [&](int64_t, unsigned u) { atomicAdd(&hist[(u >> 19) & (kSymBins - 1)], 1u); });
-  __syncthreads();
-  const unsigned b1 = (unsigned)sym_find_bin(hist, scan, kk);
-  for (int i = tid; i < kSymBins; i += 256) hist[i] = 0u;
-  __syncthreads();
-  sym_row_visit(row, n, [&](int64_t, unsigned u) {
-    if ((u >> 19) == b1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
-  });
-  __syncthreads();
-  const unsigned b2 = (unsigned)sym_find_bin(hist, scan, kk);
-
-  // U = the first float above the fine bin (b1 < 0xff0: keys are finite)
-  const unsigned ub = (((b1 << 12) | b2) + 1u) << 7;
-  const float cut = b1 >= 0xff0u ? INFINITY : __uint_as_float(ub) * (1.0f + 2.4e-7f) + eps2[q];
-  const unsigned cut_bits = __float_as_uint(cut);
   if (tid == 0) {
     s_n = 0;
+    s_ns = 0;
+    s_nt = 0;
     s_ovf = kInfBits;
   }
-  __syncthreads();
-  uint2* const out = ent + q * (int64_t)cap;
-  sym_row_visit(row, n, [&](int64_t i, unsigned u) {
-    if (u <= cut_bits) {
-      const int pos = atomicAdd(&s_n, 1);
-      if (pos < cap)
-        out[pos] = make_uint2(u, (unsigned)i);
-      else
-        atomicMin(&s_ovf, u);
+  bool done = false;
+  if (n >= kSymSampleMin) {
+    const int c = (int)(n / 256), stride = (int)(n / 16), m = 16 * c;
+    auto sample_key = [&](int t) {
+      const int j = t / c;
+      return __float_as_uint(row[(int64_t)j * stride + (t - j * c)]);
+    };
+    unsigned kr = (unsigned)min((int64_t)m - 1, 2 * (((int64_t)k1 * m + n - 1) / n) + 8);
+    clear_hist();
+    __syncthreads();
+    for (int t = tid; t < m; t += 256) atomicAdd(&hist[(sample_key(t) >> 19) & (kSymBins - 1)], 1u);
+    __syncthreads();
+    const unsigned sb1 = (unsigned)sym_find_bin(hist, scan, kr);
+    clear_hist();
+    __syncthreads();
+    for (int t = tid; t < m; t += 256) {
+      const unsigned u = sample_key(t);
+      if ((u >> 19) == sb1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
     }
-  });
+    __syncthreads();
+    const unsigned sb2 = (unsigned)sym_find_bin(hist, scan, kr);
+    const unsigned top = sym_bin_top(sb1, sb2);
+    const unsigned w_bits = sym_window_bits(top, e2);
+    sym_row_visit(row, n, [&](int64_t i, unsigned u) {
+      if (u <= w_bits) {
+        if (u < top) atomicAdd(&s_nt, 1);
+        const int p = atomicAdd(&s_ns, 1);
+        if (p < kSymStage) stage[p] = make_uint2(u, (unsigned)i);
+      }
+    });
+    __syncthreads();
+    const int ns = s_ns;
+    if (s_nt >= k1 && ns <= kSymStage) {
+      select_staged(ns, (unsigned)(k1 - 1));
+      done = true;
+    }
+    __syncthreads();
+    if (tid == 0) s_ns = 0;
+  }
+
+  if (!done) {
+    unsigned kk = (unsigned)(k1 - 1);
+    clear_hist();
+    __syncthreads();
+    sym_row_visit(row, n,
+                  [&](int64_t, unsigned u) { atomicAdd(&hist[(u >> 19) & (kSymBins - 1)], 1u); });
+    __syncthreads();
+    const unsigned b1 = (unsigned)sym_find_bin(hist, scan, kk);
+    const unsigned lo1 = b1 << 19;
+    const unsigned c1_bits = sym_window_bits((b1 + 1u) << 19, e2);
+    sym_row_visit(row, n, [&](int64_t i, unsigned u) {
+      if (u < lo1) {
+        put(u, (unsigned)i);
+      } else if (u <= c1_bits) {
+        const int p = atomicAdd(&s_ns, 1);
+        if (p < kSymStage) stage[p] = make_uint2(u, (unsigned)i);
+      }
+    });
+    clear_hist();
+    __syncthreads();
+    const int ns = s_ns;
+    if (ns <= kSymStage) {
+      for (int p = tid; p < ns; p += 256) {
+        const unsigned u = stage[p].x;
+        if ((u >> 19) == b1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
+      }
+    } else {
+      sym_row_visit(row, n, [&](int64_t, unsigned u) {
+        if ((u >> 19) == b1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
+      });
+    }
+    __syncthreads();
+    const unsigned b2 = (unsigned)sym_find_bin(hist, scan, kk);
+    const unsigned cut_bits = sym_window_bits(sym_bin_top(b1, b2), e2);
+    if (ns <= kSymStage) {
+      for (int p = tid; p < ns; p += 256) {
+        const uint2 e = stage[p];
+        if (e.x <= cut_bits) put(e.x, e.y);
+      }
+    } else {
+      sym_row_visit(row, n, [&](int64_t i, unsigned u) {
+        if (u >= lo1 && u <= cut_bits) put(u, (unsigned)i);
+      });
+    }
+  }
   __syncthreads();
   if (tid == 0) {
     cnt[q] = min(s_n, cap);

@@ -87,6 +87,33 @@ def test_symmetric_duplicates_overflow_to_exact_rows():
     assert np.abs(d[:2100]).max() < 1e-5
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_symmetric_duplicates_beyond_stage(prec):
+    """5000 equal frames: more keys than the select pass can stage in LDS
+    fall near the (k+1)-th smallest, so it reads the row again for the fine
+    histogram and the gather; outputs still equal the list path's."""
+    rng = np.random.default_rng(2)
+    base = rng.normal(size=(1, 40, 3)).astype(np.float32)
+    frames = np.repeat(base, 5100, axis=0)
+    frames[5000:] += 0.05 * rng.normal(size=(100, 40, 3)).astype(np.float32)
+    w = synth.masses(40, "uniform")
+    d, i, st = _both(prec, frames, w, 5)
+    assert np.abs(d[:5000]).max() < 1e-5
+
+
+def test_symmetric_sampled_select():
+    """Rows of 32768 keys or more take the sampled select.  34000 distinct
+    frames (the sampled bound holds and stages few keys) plus 6000 equal
+    ones (their near-zero keys overflow the LDS stage, so those rows fall
+    back to the full select and then to exact rows)."""
+    frames = synth.trajectory(40000, 16, 21)
+    frames[34000:] = frames[34000]
+    w = synth.masses(16, "uniform")
+    d, i, st = _both("bf16", frames, w, 7)
+    assert st["n_fallback_rows"] >= 6000
+    assert np.abs(d[34000:]).max() < 1e-5
+
+
 def test_symmetric_large_k():
     frames = synth.trajectory(1500, 64, 9)
     w = synth.masses(64, "uniform")
