@@ -389,7 +389,7 @@ def main():
                          "(0 = skip)")
     ap.add_argument("--fp16-steps", type=int, default=1,
                     help="steps of the split-FP16 form, reported beside the headline (0 = skip)")
-    ap.add_argument("--serving-seconds", type=float, default=30.0,
+    ap.add_argument("--serving-seconds", type=float, default=60.0,
                     help="length of the Poisson serving run (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="mean request arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=2, help="batches in flight when serving")
