@@ -13,7 +13,11 @@
 //                    32 ref x 32 fit frames), FP64 QCP Newton epilogue giving
 //                    an approximate score s~ = G_r + G_f - 2 lambda_max, and
 //                    per-lane candidate lists (threshold + append, ballot
-//                    select compaction).  No distance matrix is written.
+//                    select compaction); no distance matrix is written.
+//                    For a self query it may instead score the upper block
+//                    triangle only and write the n x n FP32 score matrix,
+//                    from which sym_select_kernel lists each row's
+//                    candidates.
 //   merge_kernel     per query: radix-select the (k+1)-th smallest s~ over
 //                    its lists, take every candidate within the certified
 //                    window, re-score them exactly in FP64 from the raw
