@@ -40,3 +40,36 @@ def test_no_device_errors_are_codes_not_crashes():
     assert L.mdsctk_knn_destroy(None) == knn.ERR_ARG
     assert L.mdsctk_knn_last_error(None) == b"null context"
     assert L.mdsctk_knn_query(None, None, 0, 1, 0, None, None) == knn.ERR_ARG
+
+
+def _header(name):
+    return open(os.path.join(ROOT, "include", name)).read()
+
+
+def test_python_constants_match_the_header():
+    """Every MDSCTK_* value the binding mirrors equals the header's."""
+    defs = {m.group(1): int(m.group(2))
+            for m in re.finditer(r"#define MDSCTK_(\w+)\s+\(?(-?\d+)u?\)?", _header("mdsctk_knn.h"))}
+    mirrored = {name: getattr(knn, name) for name in dir(knn)
+                if re.fullmatch(r"(PREC|KNN|OPT|ERR|METRIC)_\w+", name)}
+    assert mirrored, "no constants found in the binding"
+    for name, value in mirrored.items():
+        assert name in defs, name
+        assert defs[name] == value, (name, defs[name], value)
+
+
+def _struct_fields(text, name):
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    return [(t, f) for t, f in re.findall(r"(int32_t|int64_t|double)\s+(\w+);", body)]
+
+
+@pytest.mark.parametrize("cname,pycls", [("mdsctk_knn_stats", "Stats"),
+                                         ("mdsctk_knn_request_info", "RequestInfo"),
+                                         ("mdsctk_knn_serve_stats", "ServeStats")])
+def test_python_structs_match_the_header(cname, pycls):
+    """The ctypes mirrors have the header's fields, types and order."""
+    ctype = {"int32_t": ctypes.c_int32, "int64_t": ctypes.c_int64, "double": ctypes.c_double}
+    want = [(f, ctype[t]) for t, f in _struct_fields(_header("mdsctk_knn.h"), cname)]
+    got = list(getattr(knn, pycls)._fields_)
+    assert want and got == want
