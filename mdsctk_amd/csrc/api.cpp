@@ -265,6 +265,7 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
   server_stop(c);
   HIP_TRY(c, hipSetDevice(c->device));
   free_frames(c, c->ref);
+  release_score_matrix(c);
   dfree(c, c->data_shift);
   c->data_shift = nullptr;
   c->has_ref = false;
@@ -327,6 +328,7 @@ int set_data_reference_impl(mdsctk_knn_ctx* c, const double* x, int64_t n_ref, i
   server_stop(c);
   HIP_TRY(c, hipSetDevice(c->device));
   free_frames(c, c->ref);
+  release_score_matrix(c);
   dfree(c, c->data_shift);
   c->data_shift = nullptr;
   c->has_ref = false;
@@ -371,6 +373,15 @@ int slot_init(mdsctk_knn_ctx* c, Slot& s) {
   int rc;
   if ((rc = dalloc(c, (void**)&s.n_cand, 256, "scratch"))) return rc;
   return 0;
+}
+
+void release_score_matrix(mdsctk_knn_ctx* c) {
+  Slot& s = c->main;
+  if (!s.ws_sym) return;
+  if (s.st) (void)hipStreamSynchronize(s.st);
+  dfree(c, s.ws_sym);
+  s.ws_sym = nullptr;
+  s.ws_sym_bytes = 0;
 }
 
 void slot_release(mdsctk_knn_ctx* c, Slot& s) {

@@ -149,3 +149,34 @@ def test_symmetric_data_vectors(metric):
     np.testing.assert_array_equal(out[0][0], out[1][0])
     d_o, i_o = oracle.knn_data(x, x, 11, metric)
     assert np.abs(out[0][0] - d_o).max() < 1e-9
+
+
+def test_symmetric_then_cross_then_serving_on_one_context():
+    """A self query (score matrix), a cross query (lists) and a served
+    request share one context's workspace; each gives what it gives on a
+    fresh context.  Serving releases the score matrix first."""
+    frames = synth.metastable_trajectory(1500, 100, 8, n_centres=5, dup_fraction=0.1)
+    other = synth.trajectory(400, 100, 9)
+    w = synth.masses(100, "backbone")
+    d_self, i_self, _ = _self_query("bf16", frames, w, 12)
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_reference(frames, w)
+        d_cross, i_cross = ctx.query(other, 12)
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_reference(frames, w)
+        for _ in range(2):
+            d, i = ctx.query(None, 12)
+            assert ctx.stats()["n_pairs_computed"] < 1500 * 1500
+            np.testing.assert_array_equal(d, d_self)
+            np.testing.assert_array_equal(i, i_self)
+            d, i = ctx.query(other, 12)
+            np.testing.assert_array_equal(d, d_cross)
+            np.testing.assert_array_equal(i, i_cross)
+        r = ctx.wait(ctx.submit(other, 12))
+        np.testing.assert_array_equal(r.dist, d_cross)
+        np.testing.assert_array_equal(r.idx, i_cross)
+        # a new reference set after the self query
+        ctx.set_reference(other, w)
+        d, i = ctx.query(None, 5)
+        d_o, i_o = oracle.knn_exact(other, other, w, 5)
+        assert np.abs(d - d_o).max() < 1e-4
