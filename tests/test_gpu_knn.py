@@ -131,3 +131,26 @@ def test_c1_workload(prec):
     d_o, i_o = oracle.knn_exact(ref, ref, w, 30)
     err, same = check_knn(ref, ref, w, d, i, d_o, i_o)
     print(prec, "C1 max err A", err, "same-index fraction", same, st)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_c2_workload_full_size(prec):
+    """50,000 reference x 50,000 query frames (independent seed streams) x
+    304 atoms, k=100: 16 sampled rows against the FP64 oracle, and every row
+    sorted, with distinct indices in range."""
+    ref = synth.trajectory(50_000, 304, 21)
+    fit = synth.trajectory(50_000, 304, 22, base_seed=21)
+    w = synth.masses(304, "backbone")
+    k = 100
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(ref, w)
+        d, i = ctx.query(fit, k)
+        st = ctx.stats()
+    assert d.shape == (50_000, k) and st["n_fallback_rows"] == 0
+    assert np.all(np.diff(d, axis=1) >= 0)
+    srt = np.sort(i, axis=1)
+    assert np.all(srt[:, 1:] != srt[:, :-1]) and i.min() >= 0 and i.max() < 50_000
+    rows = np.random.default_rng(5).choice(50_000, 16, replace=False)
+    d_o, i_o = oracle.knn_exact(ref, fit[rows], w, k)
+    check_knn(ref, fit[rows], w, d[rows], i[rows], d_o, i_o)
+    print(prec, "C2", st["ms_total"], "ms", st["n_candidates"] / 50_000, "candidates per query")
