@@ -52,6 +52,8 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 constexpr unsigned kInfBits = 0x7f800000u;
 constexpr int kCandMax = 2048;
+constexpr int kCandSmall = 1024;   // merge instance for k1 <= kMergeSmallK1
+constexpr int kMergeSmallK1 = 256;
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -1046,15 +1048,17 @@ __device__ void block_bitonic_sort(double* s, int* idx, int n_pow2) {
 // frame from memory.
 constexpr int kMergeStageAtoms = 2048;
 
-template <bool STAGE>
+// CAND: the most candidates a row's window may hold (more: exact path).
+// The smaller instance leaves room for more workgroups per CU.
+template <bool STAGE, int CAND>
 __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
   const int64_t q = P.order ? P.order[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   __shared__ unsigned hist[256];
   __shared__ unsigned s_prefix;
   __shared__ int s_kk, s_nc, s_flag, s_total;
-  __shared__ int cand_idx[kCandMax];
-  __shared__ double cand_s[kCandMax];
+  __shared__ int cand_idx[CAND];
+  __shared__ double cand_s[CAND];
 
   const int* cnt = P.cnt + q * P.nl;
   const uint2* ent = P.ent + q * P.nl * (int64_t)P.cap;
@@ -1111,13 +1115,13 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
       const uint2 e = ent[(int64_t)l * P.cap + i];
       if (__uint_as_float(e.x) <= cut) {
         const int pos = atomicAdd(&s_nc, 1);
-        if (pos < kCandMax) cand_idx[pos] = (int)e.y;
+        if (pos < CAND) cand_idx[pos] = (int)e.y;
       }
     }
   }
   __syncthreads();
   const int nc = s_nc;
-  if (s_flag || nc > kCandMax) {
+  if (s_flag || nc > CAND) {
     if (tid == 0) P.fallback[q] = 1;
     return;
   }
@@ -1839,11 +1843,22 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
   P.fallback = fallback;
   P.n_cand = n_cand;
   P.order = order;
-  if (w.metric == kMetricRMSD && ref.natoms <= kMergeStageAtoms)
-    hipLaunchKernelGGL(merge_kernel<true>, dim3((unsigned)fit.n), dim3(256),
-                       (size_t)ref.natoms * sizeof(F3), st, P);
-  else
-    hipLaunchKernelGGL(merge_kernel<false>, dim3((unsigned)fit.n), dim3(256), 0, st, P);
+  // k1 <= 256: windows of up to 1024 candidates (C3: 184 per query), half
+  // the candidate LDS of the general instance
+  const bool stage = w.metric == kMetricRMSD && ref.natoms <= kMergeStageAtoms;
+  const size_t dyn = stage ? (size_t)ref.natoms * sizeof(F3) : 0;
+  const dim3 grid((unsigned)fit.n), block(256);
+  if (k1 <= kMergeSmallK1) {
+    if (stage)
+      hipLaunchKernelGGL((merge_kernel<true, kCandSmall>), grid, block, dyn, st, P);
+    else
+      hipLaunchKernelGGL((merge_kernel<false, kCandSmall>), grid, block, dyn, st, P);
+  } else {
+    if (stage)
+      hipLaunchKernelGGL((merge_kernel<true, kCandMax>), grid, block, dyn, st, P);
+    else
+      hipLaunchKernelGGL((merge_kernel<false, kCandMax>), grid, block, dyn, st, P);
+  }
   return hipGetLastError();
 }
 
