@@ -746,7 +746,7 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
     int cnt = 0;
     float thr = INFINITY, ovf = INFINITY;
 
-    // Stage s of this item is K-group g = s % ng3 of tile t = s / ng3.  Wave
+    // Stage s of this item is K-group g = s % ng3 of tile t = t0 + s / ng3.  Wave
     // w < 2 copies ref tile 2p+w (p = ph + t*R), wave 2..FT+1 fit tile
     // FT*fb+w-2.  Stages past the item's last tile re-read its last tile
     // (never used).
