@@ -130,7 +130,9 @@ int mdsctk_knn_set_reference_device(mdsctk_knn_ctx* ctx, const float* d_xyz_nm, 
  * Replaces the pair loops + permutation::sort + row writes
  * (knn_rms.cpp:231-291, mdsctk.h:177-199).
  *   xyz_nm   query frames [n_fit][natoms][3]; NULL queries the reference set
- *            against itself (knn_rms without -f, knn_rms.cpp:103-104).
+ *            against itself (knn_rms without -f, knn_rms.cpp:103-104), each
+ *            unordered pair scored once when MDSCTK_OPT_SELF_SYMMETRIC allows
+ *            (same outputs).
  *   k        neighbours per row; clamped to n_ref-1 (knn_rms.cpp:224-225).
  *   out_dist [n_fit][k] Angstrom, ascending; out_idx [n_fit][k] 0-based.
  *            With MDSCTK_KNN_FULL_ROWS the outputs are [n_fit][n_ref] in
