@@ -2,8 +2,8 @@
 
 Independent checks: numpy SVD-Kabsch golden vectors (tests/golden), a
 long-double Jacobi eigen-solve of the 4x4 key matrix, closed forms (rigid
-copies, the 2-atom case, symmetry, weight scaling) and the reference-
-equivalent float port."""
+copies, the 2- and 3-atom cases, symmetry, weight scaling, a small isotropic
+perturbation) and the reference-equivalent float port."""
 import os
 
 import numpy as np
@@ -150,3 +150,32 @@ def test_knn_data_rows(metric):
     # k clamps to n_ref - 1
     d, i = oracle.knn_data(x[:5], y, 30, metric)
     assert d.shape == (40, 4)
+
+
+def test_three_atom_closed_form():
+    # equilateral triangles of sides s1, s2 (equal weights, any pose): every
+    # atom sits s/sqrt(3) from the centroid, the optimal rotation aligns the
+    # two, so RMSD = |s1 - s2| / sqrt(3)
+    rng = np.random.default_rng(7)
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0.5, np.sqrt(3) / 2, 0]])
+    for s1, s2 in [(1.0, 1.0), (1.0, 1.5), (0.7, 2.3)]:
+        R1, R2 = synth.random_rotations(2, rng)
+        a = (s1 * tri @ R1.T + rng.normal(size=3)).astype(np.float32)
+        b = (s2 * tri @ R2.T + rng.normal(size=3)).astype(np.float32)
+        want = abs(s1 - s2) / np.sqrt(3.0)
+        assert abs(oracle.rmsd_nm(a, b, np.ones(3, np.float32)) - want) < 1e-6
+
+
+def test_small_isotropic_perturbation():
+    # b = a + delta with a tiny isotropic delta: the fitted RMSD never exceeds
+    # the centred-only one, and the rotation removes only about 3 of the 3N
+    # degrees of freedom of delta, so it stays within a few % of it
+    x = synth.trajectory(1, 400, 8)[0].astype(np.float64)
+    rng = np.random.default_rng(8)
+    delta = 1e-3 * rng.normal(size=x.shape)
+    a, b = x.astype(np.float32), (x + delta).astype(np.float32)
+    w = np.ones(400, np.float32)
+    fit = oracle.rmsd_nm(a, b, w)
+    nofit = oracle.rmsd_nm(a, b, w, fit=False)
+    assert fit <= nofit + 1e-12
+    assert fit >= 0.97 * nofit
