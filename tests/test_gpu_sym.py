@@ -180,3 +180,17 @@ def test_symmetric_then_cross_then_serving_on_one_context():
         d, i = ctx.query(None, 5)
         d_o, i_o = oracle.knn_exact(other, other, w, 5)
         assert np.abs(d - d_o).max() < 1e-4
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 33])
+@pytest.mark.parametrize("flags", [0, knn.KNN_KEEP_RANK0])
+def test_symmetric_tiny_sets(n, flags):
+    """Sets smaller than one tile, with the largest k each flag allows."""
+    frames = synth.trajectory(n, 24, 30 + n)
+    w = synth.masses(24, "random", n)
+    k = n if flags & knn.KNN_KEEP_RANK0 else n - 1
+    d, i, st = _both("bf16", frames, w, k, flags)
+    d_o, i_o = oracle.knn_exact(frames, frames, w, k, keep_rank0=bool(flags))
+    # the oracle clamps k to n-1 even when rank 0 is kept
+    m = d_o.shape[1]
+    check_knn(frames, frames, w, d[:, :m], i[:, :m], d_o, i_o)
