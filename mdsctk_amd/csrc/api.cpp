@@ -719,6 +719,22 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
   }
 
   float ms_contract = 0, ms_merge = 0, ms_fallback = 0;
+  // The fused pipeline runs on the slot's frames; a self query points the
+  // slot at the resident set for its duration.  A self query whose k+1 the
+  // candidate lists cannot hold still takes the symmetric form when it
+  // fits (its select list and merge hold up to 2048 candidates).
+  const bool sym_wide = !q.full && !q.fused && !xyz && c->self_sym && c->prec != kExact &&
+                        q.k1 <= kSymMaxK1;
+  int fused_rc = 1;  // 1: the fused pipeline did not run
+  FusedInfo info;
+  if (!q.full && (q.fused || sym_wide)) {
+    DevFrames saved = s.fit;
+    if (!xyz) s.fit = query_view(c->ref);
+    if (!xyz && c->self_sym) fused_rc = launch_fused_sym(c, s, q, d_dist, d_idx, &info);
+    if (fused_rc == 1 && q.fused) fused_rc = launch_fused(c, s, q, d_dist, d_idx, &info);
+    if (!xyz) s.fit = saved;
+    if (fused_rc < 0) return fused_rc;
+  }
   if (q.full) {
     HIP_TRY(c, launch_exact_rows(c->ref, *fitp, c->w, nullptr, (int)n_fit, q.nofit, nullptr,
                                  d_dist, c->ref.n, s.st));
@@ -726,7 +742,7 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
     HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
     HIP_TRY(c, hipEventSynchronize(s.ev[2]));
     ms_contract = elapsed(s.ev[1], s.ev[2]);
-  } else if (!q.fused) {
+  } else if (fused_rc == 1) {
     HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
     rc = exact_rows_select(c, s, *fitp, nullptr, (int)n_fit, q, d_dist, d_idx);
     if (rc) return rc;
@@ -735,16 +751,6 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
     ms_fallback = elapsed(s.ev[2], s.ev[3]);
     S.n_fallback_rows = n_fit;
   } else {
-    // The fused pipeline runs on the slot's frames; a self query points the
-    // slot at the resident set for its duration.
-    DevFrames saved = s.fit;
-    if (!xyz) s.fit = query_view(c->ref);
-    FusedInfo info;
-    rc = 1;
-    if (!xyz && c->self_sym) rc = launch_fused_sym(c, s, q, d_dist, d_idx, &info);
-    if (rc == 1) rc = launch_fused(c, s, q, d_dist, d_idx, &info);
-    if (!xyz) s.fit = saved;
-    if (rc) return rc;
     std::vector<int> fb(n_fit);
     int ncand = 0;
     HIP_TRY(c, hipMemcpyAsync(fb.data(), info.d_fallback, n_fit * sizeof(int),

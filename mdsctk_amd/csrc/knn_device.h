@@ -18,6 +18,9 @@ constexpr int kTileAlign = 4;
 // List capacity of a self query through the score matrix (one list per
 // row): the merge re-scores at most this many candidates per row.
 constexpr int kSymListCap = 2048;
+// Largest k+1 a self query takes through the score matrix when the
+// candidate lists cannot serve it (their limit is 896).
+constexpr int kSymMaxK1 = 1536;
 
 enum Prec : int { kFP32 = 0, kBF16 = 1, kFP16 = 2, kExact = 3 };
 

@@ -194,3 +194,25 @@ def test_symmetric_tiny_sets(n, flags):
     # the oracle clamps k to n-1 even when rank 0 is kept
     m = d_o.shape[1]
     check_knn(frames, frames, w, d[:, :m], i[:, :m], d_o, i_o)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_symmetric_wide_k_beyond_list_capacity(prec):
+    """k+1 above the candidate lists' 896: a self query still takes the
+    symmetric form (select list and merge hold 2048) and equals the
+    all-pairs FP64 path bit for bit."""
+    frames = synth.metastable_trajectory(3000, 48, 12, n_centres=4, dup_fraction=0.05)
+    w = synth.masses(48, "backbone")
+    k = 1200
+    d_s, i_s, st_s = _self_query(prec, frames, w, k)
+    d_e, i_e, st_e = _self_query(prec, frames, w, k, sym=False)
+    print(prec, "wide k", st_s["n_pairs_computed"], st_s["n_candidates"], st_s["n_fallback_rows"],
+          st_s["ms_total"], "ms vs exact", st_e["ms_total"], "ms")
+    assert st_s["n_pairs_computed"] == _triangle_pairs(3000, 64 if prec == "fp32" else 128)
+    assert st_e["n_fallback_rows"] == 3000        # the all-pairs FP64 path
+    np.testing.assert_array_equal(i_s, i_e)
+    np.testing.assert_array_equal(d_s, d_e)
+    rows = np.arange(0, 3000, 97)
+    d_o, i_o = oracle.knn_exact(frames, frames[rows], w, k)
+    # rank 0 (the frame itself) is dropped in both
+    check_knn(frames, frames[rows], w, d_s[rows], i_s[rows], d_o, i_o)
