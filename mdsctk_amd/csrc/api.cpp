@@ -590,7 +590,8 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
       return 1;
   }
   const size_t small = (size_t)n * (sizeof(int) + sizeof(float)) +
-                       (size_t)n * (sizeof(float) + sizeof(int)) + 256;
+                       (size_t)n * (sizeof(float) + sizeof(int)) +
+                       (size_t)n * 2 * sizeof(int) + 512;
   int rc = ensure(c, &s.ws_sym, &s.ws_sym_bytes, sym_bytes, "score matrix");
   if (!rc) rc = ensure(c, &s.ws_lists, &s.ws_lists_bytes, ent_bytes, "candidate lists");
   if (!rc) rc = ensure(c, &s.ws_small, &s.ws_small_bytes, small, "list headers");
@@ -608,9 +609,12 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
   L.ent = (uint2*)s.ws_lists;
   char* sp = (char*)s.ws_small;
   double* gmax_q = (double*)sp; sp += 256;
+  int* counts = (int*)sp; sp += 256;
   L.cnt = (int*)sp; sp += n * sizeof(int);
   L.ovf = (float*)sp; sp += n * sizeof(float);
   float* eps2 = (float*)sp; sp += n * sizeof(float);
+  int* label = (int*)sp; sp += n * sizeof(int);
+  int* order = (int*)sp; sp += n * sizeof(int);
   int* d_fallback = (int*)sp;
   const double kappa = (c->w.metric == kMetricRMSD ? kappa_for(c->prec, c->ref.natoms)
                                                    : kappa_data(c->prec, c->ref.natoms)) *
@@ -624,8 +628,12 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
                              sym, ld));
   HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
   HIP_TRY(c, launch_sym_select(sym, ld, n, eps2, q.k1, L, s.st));
+  // merge rows grouped by landmark (it pays where candidate frames no
+  // longer fit the caches; launch_sym_order needs n >= 64)
+  const bool ordered = n >= 512;
+  if (ordered) HIP_TRY(c, launch_sym_order(sym, ld, n, label, counts, order, s.st));
   HIP_TRY(c, launch_merge(c->ref, fit, c->w, L, eps2, q.k1, q.k_out, q.rank_off, q.nofit, d_dist,
-                          d_idx, d_fallback, s.n_cand, nullptr, s.st));
+                          d_idx, d_fallback, s.n_cand, ordered ? order : nullptr, s.st));
   HIP_TRY(c, hipEventRecord(s.ev[4], s.st));
   // pairs of the upper block triangle: fit block fb against frames from
   // its own first frame on

@@ -135,6 +135,11 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
 // G~ref[r] + max G~q); *gmax_q receives max G~q.
 hipError_t launch_eps_sym(const DevFrames& fit, const DevFrames& ref, double kappa,
                           double* gmax_q, float* eps2, int* n_cand, hipStream_t st);
+// Merge order for a self query through the score matrix (n >= 64): rows
+// grouped by their nearest of 64 evenly spaced frames.  label and order
+// hold n ints, counts 64.
+hipError_t launch_sym_order(const float* sym, int64_t sym_ld, int64_t n, int* label,
+                            int* counts, int* order, hipStream_t st);
 // Candidate lists (L.nl == 1) from the n x n score matrix of a self query:
 // per row, every key up to the window of its k1-th smallest key.
 hipError_t launch_sym_select(const float* sym, int64_t sym_ld, int64_t n, const float* eps2,

@@ -1405,6 +1405,61 @@ __global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict
   }
 }
 
+// Processing order for the merge of a symmetric self query: rows grouped by
+// the nearest of kSymLandmarks evenly spaced reference frames (read from the
+// score matrix), so that rows merged at the same time share their
+// candidates' frames in L2 and MALL.  The order only changes which
+// workgroup takes which row, never a result.
+constexpr int kSymLandmarks = 64;
+
+__global__ void sym_order_zero_kernel(int* __restrict__ counts) {
+  if (threadIdx.x < kSymLandmarks) counts[threadIdx.x] = 0;
+}
+
+__global__ void sym_label_kernel(const float* __restrict__ S, int64_t ld, int64_t n,
+                                 int* __restrict__ label, int* __restrict__ counts) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int64_t stride = n / kSymLandmarks;
+  const float* const row = S + q * ld;
+  unsigned best = 0xffffffffu;
+  int arg = 0;
+  for (int j = 0; j < kSymLandmarks; ++j) {
+    const unsigned u = __float_as_uint(row[j * stride]);
+    if (u < best) {
+      best = u;
+      arg = j;
+    }
+  }
+  label[q] = arg;
+  atomicAdd(&counts[arg], 1);
+}
+
+// counts -> exclusive offsets, in place (one block of kSymLandmarks threads)
+__global__ void sym_order_scan_kernel(int* __restrict__ counts) {
+  __shared__ int c[kSymLandmarks];
+  const int t = threadIdx.x;
+  c[t] = counts[t];
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int j = 0; j < kSymLandmarks; ++j) {
+      const int v = c[j];
+      c[j] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  counts[t] = c[t];
+}
+
+__global__ void sym_order_scatter_kernel(const int* __restrict__ label, int64_t n,
+                                         int* __restrict__ offsets, int* __restrict__ order) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  order[atomicAdd(&offsets[label[q]], 1)] = (int)q;
+}
+
 // ---------------------------------------------------------------------------
 // exact rows
 
@@ -1859,6 +1914,19 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
     else
       hipLaunchKernelGGL((merge_kernel<false, kCandMax>), grid, block, dyn, st, P);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_sym_order(const float* sym, int64_t sym_ld, int64_t n, int* label,
+                            int* counts, int* order, hipStream_t st) {
+  if (n < kSymLandmarks || sym_ld < n) return hipErrorInvalidValue;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(sym_order_zero_kernel, dim3(1), dim3(kSymLandmarks), 0, st, counts);
+  hipLaunchKernelGGL(sym_label_kernel, dim3(blocks), dim3(256), 0, st, sym, sym_ld, n, label,
+                     counts);
+  hipLaunchKernelGGL(sym_order_scan_kernel, dim3(1), dim3(kSymLandmarks), 0, st, counts);
+  hipLaunchKernelGGL(sym_order_scatter_kernel, dim3(blocks), dim3(256), 0, st, label, n, counts,
+                     order);
   return hipGetLastError();
 }
 
