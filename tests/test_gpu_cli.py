@@ -47,6 +47,22 @@ def test_knn_rms_self_query_files(tmp_path, trp, prec):
     check_knn(x, x, w, d, i, *oracle.knn_exact(x, x, w, 12))
 
 
+def test_knn_rms_self_query_forms_agree(tmp_path, trp):
+    """-f omitted, -f equal to -r (both a self query through the score
+    matrix) and -f naming a copy of the file (a cross query through the
+    candidate lists) write identical files."""
+    import shutil
+    copy = tmp_path / "copy.xtc"
+    shutil.copy(TRP_XTC, copy)
+    outs = []
+    for tag, extra in (("a", []), ("b", ["-f", TRP_XTC]), ("c", ["-f", str(copy)])):
+        p = run(build.CLI_PATH, ["-k", "15", "-p", TRP_PDB, "-r", TRP_XTC, "-d", f"d{tag}.dat",
+                                 "-i", f"i{tag}.dat"] + extra, tmp_path)
+        assert p.returncode == 0, p.stdout + p.stderr
+        outs.append(((tmp_path / f"d{tag}.dat").read_bytes(), (tmp_path / f"i{tag}.dat").read_bytes()))
+    assert outs[0] == outs[1] == outs[2]
+
+
 def test_knn_rms_out_of_sample_and_nofit(tmp_path, trp):
     x, y, w = trp
     p = run(build.CLI_PATH, ["-k", "8", "-p", TRP_PDB, "-r", TRP_XTC, "-f", TRP_OOS, "-n", "true",
