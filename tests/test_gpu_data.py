@@ -156,6 +156,24 @@ def run(args, cwd):
 
 
 @pytest.mark.parametrize("corr", [False, True])
+def test_knn_data_cli_self_query_forms_agree(tmp_path, corr):
+    """knn_data without -f (a self query through the score matrix) and with
+    -f naming a copy of the reference file (a cross query through the
+    candidate lists) write identical files."""
+    ref = synth.data_vectors(700, 21, 13, "clusters")
+    ref.tofile(tmp_path / "reference.pts")
+    ref.tofile(tmp_path / "copy.pts")
+    extra = ["-c"] if corr else []
+    outs = []
+    for tag, more in (("a", []), ("b", ["-f", "copy.pts"])):
+        p = run(["-k", "9", "-v", "21", "-d", f"d{tag}.dat", "-i", f"i{tag}.dat"] + more + extra,
+                tmp_path)
+        assert p.returncode == 0, p.stdout + p.stderr
+        outs.append(((tmp_path / f"d{tag}.dat").read_bytes(), (tmp_path / f"i{tag}.dat").read_bytes()))
+    assert outs[0] == outs[1]
+
+
+@pytest.mark.parametrize("corr", [False, True])
 def test_knn_data_cli(tmp_path, corr):
     ref = synth.data_vectors(400, 33, 9, "dihedral")
     fit = synth.data_vectors(90, 33, 10, "dihedral")
