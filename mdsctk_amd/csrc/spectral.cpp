@@ -9,10 +9,14 @@
 // decomp_sparse.cpp:236-264 with the same subspace size and target.
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/mdsctk_graph.h"
@@ -117,15 +121,167 @@ double rand_unit(uint64_t& s) {
 
 // w -= V[:, 0..j] (V^T w), twice; returns the first-pass coefficients plus
 // the second-pass corrections in h[0..j].
-void reorthogonalise(const std::vector<double>& V, int32_t n, int j, double* w, double* h) {
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int i = 0; i <= j; ++i) {
-      const double c = dot(&V[(size_t)i * n], w, n);
-      if (pass == 0) h[i] = c; else h[i] += c;
-      const double* vi = &V[(size_t)i * n];
-      for (int32_t r = 0; r < n; ++r) w[r] -= c * vi[r];
+// A fixed team of host threads for the Lanczos vector work: run(f) calls
+// f(t) on members t = 0..size()-1 (the caller is member 0) and returns when
+// all are done; barrier() synchronises the members inside run().
+class Team {
+ public:
+  explicit Team(int size) : size_(std::max(1, size)) {
+    for (int t = 1; t < size_; ++t) workers_.emplace_back([this, t] { work(t); });
+  }
+  ~Team() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& th : workers_) th.join();
+  }
+  int size() const { return size_; }
+  void run(const std::function<void(int)>& f) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      job_ = &f;
+      done_ = 0;
+      ++gen_;
+    }
+    cv_.notify_all();
+    f(0);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [&] { return done_ == size_ - 1; });
+    job_ = nullptr;
+  }
+  void barrier() {
+    std::unique_lock<std::mutex> lk(mu_);
+    const uint64_t phase = bar_phase_;
+    if (++bar_count_ == size_) {
+      bar_count_ = 0;
+      ++bar_phase_;
+      bar_cv_.notify_all();
+    } else {
+      bar_cv_.wait(lk, [&] { return bar_phase_ != phase; });
     }
   }
+
+ private:
+  void work(int t) {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(int)>* f;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        f = job_;
+      }
+      (*f)(t);
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        ++done_;
+      }
+      done_cv_.notify_one();
+    }
+  }
+  const int size_;
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_, bar_cv_;
+  const std::function<void(int)>* job_ = nullptr;
+  uint64_t gen_ = 0, bar_phase_ = 0;
+  int done_ = 0, bar_count_ = 0;
+  bool stop_ = false;
+};
+
+// Row blocks of the reorthogonalisation.  Their number is fixed, so the
+// floating-point sums do not depend on how many threads take them.
+constexpr int kRowBlocks = 16;
+
+// w -= V[:, 0..j] (V[:, 0..j]^T w), twice (classical Gram-Schmidt applied
+// twice); h[0..j] receives the summed coefficients.  Rows are split into
+// kRowBlocks blocks: each block's partial inner products are formed by one
+// member, summed over the blocks in block order, and each member then
+// updates its own blocks' rows.  part holds kRowBlocks x (j+1) doubles.
+void reorthogonalise(Team& team, const std::vector<double>& V, int32_t n, int j, double* w,
+                     double* h, std::vector<double>& part) {
+  const int nv = j + 1;
+  part.resize((size_t)kRowBlocks * nv);
+  team.run([&](int t) {
+    const int T = team.size();
+    std::vector<double> hp(nv);
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int b = t; b < kRowBlocks; b += T) {
+        const int32_t r0 = (int32_t)((int64_t)n * b / kRowBlocks);
+        const int32_t r1 = (int32_t)((int64_t)n * (b + 1) / kRowBlocks);
+        for (int i = 0; i < nv; ++i)
+          part[(size_t)b * nv + i] = dot(&V[(size_t)i * n + r0], w + r0, r1 - r0);
+      }
+      team.barrier();
+      for (int i = 0; i < nv; ++i) {
+        double c = 0.0;
+        for (int b = 0; b < kRowBlocks; ++b) c += part[(size_t)b * nv + i];
+        hp[i] = c;
+      }
+      team.barrier();  // every member has read part before it is rewritten
+      for (int b = t; b < kRowBlocks; b += T) {
+        const int32_t r0 = (int32_t)((int64_t)n * b / kRowBlocks);
+        const int32_t r1 = (int32_t)((int64_t)n * (b + 1) / kRowBlocks);
+        for (int i = 0; i < nv; ++i) {
+          const double c = hp[i];
+          const double* vi = &V[(size_t)i * n];
+          for (int32_t r = r0; r < r1; ++r) w[r] -= c * vi[r];
+        }
+      }
+      if (t == 0)
+        for (int i = 0; i < nv; ++i) h[i] = pass == 0 ? hp[i] : h[i] + hp[i];
+    }
+  });
+}
+
+// Rows [r0, r1) of row block b.
+inline int32_t block_row(int32_t n, int b) { return (int32_t)((int64_t)n * b / kRowBlocks); }
+
+// The symmetric matrix whose strict lower triangle is (pcol, irow, a), as
+// full rows: row r lists (c, value) for both triangles.
+struct SymRows {
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> col;
+  std::vector<double> val;
+};
+
+SymRows sym_rows(int32_t n, const int32_t* pcol, const int32_t* irow, const double* a) {
+  SymRows m;
+  m.ptr.assign((size_t)n + 1, 0);
+  for (int32_t c = 0; c < n; ++c)
+    for (int32_t e = pcol[c]; e < pcol[c + 1]; ++e) {
+      ++m.ptr[(size_t)irow[e] + 1];
+      ++m.ptr[(size_t)c + 1];
+    }
+  for (int32_t r = 0; r < n; ++r) m.ptr[r + 1] += m.ptr[r];
+  m.col.resize(m.ptr[n]);
+  m.val.resize(m.ptr[n]);
+  std::vector<int64_t> at(m.ptr.begin(), m.ptr.end() - 1);
+  for (int32_t c = 0; c < n; ++c)
+    for (int32_t e = pcol[c]; e < pcol[c + 1]; ++e) {
+      const int32_t r = irow[e];
+      m.col[at[r]] = c;
+      m.val[at[r]++] = a[e];
+      m.col[at[c]] = r;
+      m.val[at[c]++] = a[e];
+    }
+  return m;
+}
+
+// y = S x, rows in blocks over the team.
+void sym_rows_spmv(Team& team, int32_t n, const SymRows& m, const double* x, double* y) {
+  team.run([&](int t) {
+    for (int b = t; b < kRowBlocks; b += team.size())
+      for (int32_t r = block_row(n, b); r < block_row(n, b + 1); ++r) {
+        double acc = 0.0;
+        for (int64_t e = m.ptr[r]; e < m.ptr[r + 1]; ++e) acc += m.val[e] * x[m.col[e]];
+        y[r] = acc;
+      }
+  });
 }
 
 }  // namespace
@@ -289,14 +445,18 @@ int leading_eigenpairs(int32_t n, const int32_t* pcol, const int32_t* irow,
   // Thick-restart Lanczos.
   const int m = (int)std::min<int64_t>((int64_t)10 * nev + 1, n);
   const double tol = 1e-13;
-  std::vector<double> V((size_t)(m + 1) * n), T((size_t)m * m, 0.0), h(m + 1), w(n);
+  std::vector<double> V((size_t)(m + 1) * n), T((size_t)m * m, 0.0), h(m + 1), w(n), part;
+  // threads for the vector work; small graphs are not worth them
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  Team team(n >= 4096 ? std::min(hw, kRowBlocks) : 1);
+  const SymRows S = sym_rows(n, pcol, irow, a.data());
   uint64_t seed = 0x5eed5eedull;
   auto random_start = [&](int j) {
     // a random vector orthogonal to V[:, 0..j-1], normalised into V[:, j]
     double* v = &V[(size_t)j * n];
     for (int attempt = 0; attempt < 8; ++attempt) {
       for (int32_t r = 0; r < n; ++r) v[r] = rand_unit(seed);
-      if (j > 0) reorthogonalise(V, n, j - 1, v, h.data());
+      if (j > 0) reorthogonalise(team, V, n, j - 1, v, h.data(), part);
       const double nv = std::sqrt(dot(v, v, n));
       if (nv > 1e-8) {
         for (int32_t r = 0; r < n; ++r) v[r] /= nv;
@@ -314,8 +474,8 @@ int leading_eigenpairs(int32_t n, const int32_t* pcol, const int32_t* irow,
   for (int cycle = 0; cycle < max_cycles; ++cycle) {
     int mm = m;
     for (int j = j0; j < m; ++j) {
-      sym_spmv(n, pcol, irow, a.data(), &V[(size_t)j * n], w.data());
-      reorthogonalise(V, n, j, w.data(), h.data());
+      sym_rows_spmv(team, n, S, &V[(size_t)j * n], w.data());
+      reorthogonalise(team, V, n, j, w.data(), h.data(), part);
       for (int i = 0; i <= j; ++i) T[(size_t)i * m + j] = T[(size_t)j * m + i] = h[i];
       beta = std::sqrt(dot(w.data(), w.data(), n));
       if (j + 1 == m) break;
@@ -359,14 +519,19 @@ int leading_eigenpairs(int32_t n, const int32_t* pcol, const int32_t* irow,
     // residual direction.
     const int keep = std::min(m - 1, nev + std::max(nev, (m - nev) / 2));
     std::vector<double> U((size_t)keep * n, 0.0);
-    for (int i = 0; i < keep; ++i) {
-      double* u = &U[(size_t)i * n];
-      for (int r = 0; r < mm; ++r) {
-        const double y = Y[(size_t)r * mm + i];
-        const double* vr = &V[(size_t)r * n];
-        for (int32_t q = 0; q < n; ++q) u[q] += y * vr[q];
+    team.run([&](int t) {
+      for (int b = t; b < kRowBlocks; b += team.size()) {
+        const int32_t q0 = block_row(n, b), q1 = block_row(n, b + 1);
+        for (int i = 0; i < keep; ++i) {
+          double* u = &U[(size_t)i * n];
+          for (int r = 0; r < mm; ++r) {
+            const double y = Y[(size_t)r * mm + i];
+            const double* vr = &V[(size_t)r * n];
+            for (int32_t q = q0; q < q1; ++q) u[q] += y * vr[q];
+          }
+        }
       }
-    }
+    });
     std::copy(U.begin(), U.end(), V.begin());
     double* vk = &V[(size_t)keep * n];
     for (int32_t r = 0; r < n; ++r) vk[r] = w[r] / beta;
