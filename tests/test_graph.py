@@ -91,6 +91,26 @@ def test_spectral_matches_arpack_on_trp_cage():
         assert min(np.abs(a - b).max(), np.abs(a + b).max()) < 1e-6
 
 
+def test_spectral_matches_arpack_threaded():
+    """A 12,000-node kNN graph: large enough that the solver splits its
+    vector work over host threads; eigenpairs as scipy's ARPACK gives."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(12000, 6))
+    dd, ii = cKDTree(x).query(x, 17)
+    d, i = dd[:, 1:].copy(), ii[:, 1:].astype(np.int32)
+    m = graph.ssm_build(d, i)
+    sigma = float(np.median(d))
+    ev, vec, res, got = graph.spectral_decomp(m, sigma, 8)
+    w, z = og.spectral_decomp(m.n, m.pcol, m.irow, m.vals, sigma, 8)
+    assert got == 8
+    assert np.abs(ev - w).max() < 1e-10, (ev, w)
+    assert res.max() < 1e-9
+    # the same answer again, bit for bit (fixed row blocks, fixed sum order)
+    ev2, vec2, _, _ = graph.spectral_decomp(m, sigma, 8)
+    assert (ev2 == ev).all() and (vec2 == vec).all()
+
+
 def test_spectral_closed_forms():
     # two disconnected rings of 30 and 41 nodes with equal weights: the
     # normalised affinity of a ring of n nodes has eigenvalues cos(2 pi j / n),
