@@ -793,6 +793,14 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
   }
   HIP_TRY(c, hipEventRecord(s.ev[5], s.st));
   HIP_TRY(c, hipEventSynchronize(s.ev[5]));
+  // While serving runs, a self query does not keep its score matrix: the
+  // memory goes back to the serving slots' workspace.
+  bool serving;
+  {
+    std::lock_guard<std::mutex> g(c->server_mu);
+    serving = c->server != nullptr;
+  }
+  if (serving) release_score_matrix(c);
   S.ms_prep = elapsed(s.ev[0], s.ev[1]);
   S.ms_contract = ms_contract;
   S.ms_merge = ms_merge;

@@ -120,8 +120,9 @@ int ensure(mdsctk_knn_ctx* c, void** p, size_t* have, size_t need, const char* w
 int slot_init(mdsctk_knn_ctx* c, Slot& s);
 void slot_release(mdsctk_knn_ctx* c, Slot& s);
 // Free the self query's score matrix, which the context otherwise keeps
-// for the next self query (called when the reference set changes and when
-// serving starts, so that the serving slots get that memory).
+// for the next self query.  Called by the synchronous entry points only (a
+// new reference set; the end of a self query while serving runs), so it
+// never races a query that uses the matrix.
 void release_score_matrix(mdsctk_knn_ctx* c);
 
 // Validate k/flags against the resident set and fill `q`.

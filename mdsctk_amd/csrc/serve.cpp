@@ -593,7 +593,6 @@ int server_submit(mdsctk_knn_ctx* c, const float* xyz, int64_t n_fit, int32_t k,
   int rc = resolve_shape(c, k, flags, &q);
   if (rc) return rc;
   if (!c->server) {
-    release_score_matrix(c);
     auto* s = new Server(c);
     rc = s->start();
     if (rc) {

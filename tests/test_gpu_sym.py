@@ -154,7 +154,8 @@ def test_symmetric_data_vectors(metric):
 def test_symmetric_then_cross_then_serving_on_one_context():
     """A self query (score matrix), a cross query (lists) and a served
     request share one context's workspace; each gives what it gives on a
-    fresh context.  Serving releases the score matrix first."""
+    fresh context.  While serving runs, a self query frees its score matrix
+    when it ends."""
     frames = synth.metastable_trajectory(1500, 100, 8, n_centres=5, dup_fraction=0.1)
     other = synth.trajectory(400, 100, 9)
     w = synth.masses(100, "backbone")
@@ -175,6 +176,12 @@ def test_symmetric_then_cross_then_serving_on_one_context():
         r = ctx.wait(ctx.submit(other, 12))
         np.testing.assert_array_equal(r.dist, d_cross)
         np.testing.assert_array_equal(r.idx, i_cross)
+        # a self query while serving runs, then another request
+        d, i = ctx.query(None, 12)
+        np.testing.assert_array_equal(d, d_self)
+        np.testing.assert_array_equal(i, i_self)
+        r = ctx.wait(ctx.submit(other, 12))
+        np.testing.assert_array_equal(r.dist, d_cross)
         # a new reference set after the self query
         ctx.set_reference(other, w)
         d, i = ctx.query(None, 5)
