@@ -1049,9 +1049,10 @@ __device__ void block_bitonic_sort(double* s, int* idx, int n_pow2) {
 constexpr int kMergeStageAtoms = 2048;
 
 // CAND: the most candidates a row's window may hold (more: exact path).
-// The smaller instance leaves room for more workgroups per CU.
+// The smaller instance leaves room for more workgroups per CU; the
+// waves-per-EU hint keeps it within 96 VGPRs (5 workgroups per CU).
 template <bool STAGE, int CAND>
-__global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void merge_kernel(MergeParams P) {
   const int64_t q = P.order ? P.order[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   __shared__ unsigned hist[256];
@@ -1059,12 +1060,14 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
   __shared__ int s_kk, s_nc, s_flag, s_total;
   __shared__ int cand_idx[CAND];
   __shared__ double cand_s[CAND];
+  __shared__ float cand_key[CAND];
+  __shared__ int s_bad;
 
   const int* cnt = P.cnt + q * P.nl;
   const uint2* ent = P.ent + q * P.nl * (int64_t)P.cap;
   const float* ovf = P.ovf + q * P.nl;
 
-  if (tid == 0) { s_total = 0; s_nc = 0; s_flag = 0; }
+  if (tid == 0) { s_total = 0; s_nc = 0; s_flag = 0; s_bad = 0; }
   __syncthreads();
   {
     int part = 0;
@@ -1115,7 +1118,10 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
       const uint2 e = ent[(int64_t)l * P.cap + i];
       if (__uint_as_float(e.x) <= cut) {
         const int pos = atomicAdd(&s_nc, 1);
-        if (pos < CAND) cand_idx[pos] = (int)e.y;
+        if (pos < CAND) {
+          cand_idx[pos] = (int)e.y;
+          cand_key[pos] = __uint_as_float(e.x);
+        }
       }
     }
   }
@@ -1138,6 +1144,23 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeParams P) {
   for (int c = wave; c < nc; c += 4) {
     const double s = wave_pair_score<STAGE>(P.src, cand_idx[c], q, P.nofit);
     if (lane == 0) cand_s[c] = s;
+  }
+  __syncthreads();
+  // The window assumes that every approximate score lies within
+  // eps = eps2 / 2 of the exact one.  The candidates' exact scores are known
+  // now: a row where one deviates further takes the exact path.  The keys
+  // are rounded to FP32, hence the relative allowance.
+  {
+    const double eps = 0.5 * (double)P.eps2[q];
+    for (int c = tid; c < nc; c += 256) {
+      const double s = cand_s[c], kq = (double)cand_key[c];
+      if (fabs(kq - s) > eps + 1.2e-7 * fmax(kq, s)) s_bad = 1;
+    }
+  }
+  __syncthreads();
+  if (s_bad) {
+    if (tid == 0) P.fallback[q] = 1;
+    return;
   }
   int np2 = 64;
   while (np2 < nc) np2 <<= 1;

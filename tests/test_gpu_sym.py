@@ -223,3 +223,29 @@ def test_symmetric_wide_k_beyond_list_capacity(prec):
     d_o, i_o = oracle.knn_exact(frames, frames[rows], w, k)
     # rank 0 (the frame itself) is dropped in both
     check_knn(frames, frames[rows], w, d_s[rows], i_s[rows], d_o, i_o)
+
+
+@pytest.mark.parametrize("sym", [1, 0])
+def test_run_time_window_check_catches_a_narrow_window(sym):
+    """With the candidate window shrunk far below the measured deviation of
+    the approximate scores (MDSCTK_OPT_EPS_SCALE), the merge's check of the
+    re-scored candidates sends rows to the exact path, so the outputs stay
+    exact; at the default window no row trips it."""
+    frames = synth.metastable_trajectory(1200, 200, 17, n_centres=4, dup_fraction=0.1)
+    w = synth.masses(200, "backbone")
+    k = 10
+    d_o, i_o = oracle.knn_exact(frames, frames, w, k)
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_option(knn.OPT_SELF_SYMMETRIC, sym)
+        ctx.set_reference(frames, w)
+        d, i = ctx.query(None, k)
+        assert ctx.stats()["n_fallback_rows"] == 0
+        ctx.set_option(knn.OPT_EPS_SCALE, 1e-4)
+        d2, i2 = ctx.query(None, k)
+        st = ctx.stats()
+    print("narrow window: fallback rows", st["n_fallback_rows"], "of 1200")
+    assert st["n_fallback_rows"] > 0
+    check_knn(frames, frames, w, d, i, d_o, i_o)
+    check_knn(frames, frames, w, d2, i2, d_o, i_o)
+    np.testing.assert_array_equal(d2, d)
+    np.testing.assert_array_equal(i2, i)
