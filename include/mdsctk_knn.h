@@ -64,7 +64,10 @@ extern "C" {
 
 /* Options for mdsctk_knn_set_option. */
 #define MDSCTK_OPT_HBM_BUDGET 1   /* bytes of device memory the context may hold (0 = no limit) */
-#define MDSCTK_OPT_EPS_SCALE 2    /* multiplier on the candidate window (default 1) */
+#define MDSCTK_OPT_EPS_SCALE 2    /* multiplier on the candidate window (default 1); rows whose
+                                     re-scored candidates deviate by more than the scaled
+                                     window's bound are recomputed exactly, so a scale below 1
+                                     costs time, not exactness, as far as the candidates show */
 #define MDSCTK_OPT_REF_SPLITS 3   /* force the number of reference phases (0 = automatic) */
 #define MDSCTK_OPT_SERVE_SLOTS 4  /* batches in flight at once when serving (1..4, default 2) */
 #define MDSCTK_OPT_SERVE_MAX_FRAMES 5 /* query frames per served batch (default 8192) */
