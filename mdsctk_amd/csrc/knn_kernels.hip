@@ -1181,7 +1181,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
 // candidate-list form the merge reads (one list per row).
 
 constexpr int kSymBins = 4096;
-constexpr int kSymStage = 4096;  // keys staged in LDS by the second pass
+constexpr int kSymStage = 4096;       // keys staged in LDS by the second pass
+constexpr int kSymStageSmall = 2048;  // the instance for k1 <= kMergeSmallK1
 
 // Calls f(index, key bits) for every key of a row (256 threads): 16-byte
 // loads, four in flight per thread, over the aligned part, then the tail.
@@ -1275,13 +1276,14 @@ constexpr int64_t kSymSampleMin = 32768;  // rows at least this long sample firs
 // many keys): a histogram of the whole row finds tau's coarse bin b1; a
 // second pass lists the keys below b1 (all <= tau) and stages those from b1
 // up to the window of b1's top; the fine bin comes from the staged keys
-// (or, past kSymStage of them, from a third pass over the row, which then
+// (or, past STAGE_N of them, from a third pass over the row, which then
 // also gathers).
 //
 // Every key up to the window of tau's bin top joins the list, so the list
 // holds the merge's whole window.  Past `cap` entries the smallest dropped
 // key goes to ovf, which makes the merge take the exact path if it falls
 // inside the window.
+template <int STAGE_N>
 __global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict__ S, int64_t ld,
                                                          int64_t n, const float* __restrict__ eps2,
                                                          int k1, uint2* __restrict__ ent,
@@ -1289,7 +1291,7 @@ __global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict
                                                          float* __restrict__ ovf, int cap) {
   __shared__ unsigned hist[kSymBins];
   __shared__ unsigned scan[256];
-  __shared__ uint2 stage[kSymStage];
+  __shared__ uint2 stage[STAGE_N];
   __shared__ int s_n, s_ns, s_nt;
   __shared__ unsigned s_ovf;
   const int64_t q = blockIdx.x;
@@ -1363,12 +1365,12 @@ __global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict
       if (u <= w_bits) {
         if (u < top) atomicAdd(&s_nt, 1);
         const int p = atomicAdd(&s_ns, 1);
-        if (p < kSymStage) stage[p] = make_uint2(u, (unsigned)i);
+        if (p < STAGE_N) stage[p] = make_uint2(u, (unsigned)i);
       }
     });
     __syncthreads();
     const int ns = s_ns;
-    if (s_nt >= k1 && ns <= kSymStage) {
+    if (s_nt >= k1 && ns <= STAGE_N) {
       select_staged(ns, (unsigned)(k1 - 1));
       done = true;
     }
@@ -1391,13 +1393,13 @@ __global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict
         put(u, (unsigned)i);
       } else if (u <= c1_bits) {
         const int p = atomicAdd(&s_ns, 1);
-        if (p < kSymStage) stage[p] = make_uint2(u, (unsigned)i);
+        if (p < STAGE_N) stage[p] = make_uint2(u, (unsigned)i);
       }
     });
     clear_hist();
     __syncthreads();
     const int ns = s_ns;
-    if (ns <= kSymStage) {
+    if (ns <= STAGE_N) {
       for (int p = tid; p < ns; p += 256) {
         const unsigned u = stage[p].x;
         if ((u >> 19) == b1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
@@ -1410,7 +1412,7 @@ __global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict
     __syncthreads();
     const unsigned b2 = (unsigned)sym_find_bin(hist, scan, kk);
     const unsigned cut_bits = sym_window_bits(sym_bin_top(b1, b2), e2);
-    if (ns <= kSymStage) {
+    if (ns <= STAGE_N) {
       for (int p = tid; p < ns; p += 256) {
         const uint2 e = stage[p];
         if (e.x <= cut_bits) put(e.x, e.y);
@@ -1957,8 +1959,14 @@ hipError_t launch_sym_select(const float* sym, int64_t sym_ld, int64_t n, const 
                              int k1, const Lists& L, hipStream_t st) {
   if (L.nl != 1 || k1 < 1 || k1 > n || sym_ld < n || (sym_ld & 3)) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(sym_select_kernel, dim3((unsigned)n), dim3(256), 0, st, sym, sym_ld, n, eps2,
-                     k1, L.ent, L.cnt, L.ovf, L.cap);
+  // small k: a 2048-key stage (C3 stages about 300 keys a row), so that
+  // 4 workgroups fit per CU instead of 3
+  if (k1 <= kMergeSmallK1)
+    hipLaunchKernelGGL(sym_select_kernel<kSymStageSmall>, dim3((unsigned)n), dim3(256), 0, st, sym,
+                       sym_ld, n, eps2, k1, L.ent, L.cnt, L.ovf, L.cap);
+  else
+    hipLaunchKernelGGL(sym_select_kernel<kSymStage>, dim3((unsigned)n), dim3(256), 0, st, sym,
+                       sym_ld, n, eps2, k1, L.ent, L.cnt, L.ovf, L.cap);
   return hipGetLastError();
 }
 
