@@ -129,7 +129,7 @@ def run_serving(knn, synth, oracle, prec, args):
     keep = set(rng.choice(len(arrivals), size=min(8, len(arrivals)), replace=False).tolist())
     log(f"serving: {len(arrivals)} requests over {args.serving_seconds:.0f} s, "
         f"{sizes.sum()} frames ({sizes.mean():.0f} per request)")
-    lat, done_t, kept = [], [], {}
+    lat, lat_frames, done_t, kept = [], [], [], {}
     with knn.KnnContext(0, prec) as ctx:
         ctx.set_option(knn.OPT_SERVE_GRAPHS, args.serve_graphs)
         ctx.set_option(knn.OPT_SERVE_SHARE_CUS, args.serve_share_cus)
@@ -148,6 +148,7 @@ def run_serving(knn, synth, oracle, prec, args):
                     ctx.wait(r)
                 if r.done or ctx.poll(r):
                     lat.append(r.info["t_done_ms"] - r.t_arrival)
+                    lat_frames.append(int(sizes[j]))
                     done_t.append(r.info["t_done_ms"])
                     if j in keep:
                         kept[j] = (r.dist, r.idx, r.info["batch_frames"])
@@ -185,7 +186,17 @@ def run_serving(knn, synth, oracle, prec, args):
         d_o, _ = oracle.knn_exact(res, q, w, kq)
         err = max(err, float(np.abs(d[:1] - d_o).max()) / 10.0)
     lat = np.array(lat)
+    lat_frames = np.array(lat_frames)
     span_s = (max(done_t) - t0) / 1e3
+    # latency by request size; 900..1100 frames is the "1k-frame query"
+    buckets = {}
+    for lo, hi in ((1, 16), (17, 256), (257, 899), (900, 1100), (1101, 4096)):
+        sel = lat[(lat_frames >= lo) & (lat_frames <= hi)]
+        if len(sel):
+            buckets[f"{lo}-{hi}"] = {"requests": int(len(sel)),
+                                     "p50_ms": float(np.percentile(sel, 50)),
+                                     "p99_ms": float(np.percentile(sel, 99)),
+                                     "max_ms": float(sel.max())}
     frames = float(sizes.sum())
     return {"workload": f"resident {n_res} x {na_res} atoms, Poisson {args.serving_rate:.0f} req/s, "
                         f"1..4096 frames log-uniform, k={kq}",
@@ -193,6 +204,8 @@ def run_serving(knn, synth, oracle, prec, args):
             "frames_per_request": frames / len(lat),
             "p50_ms": float(np.percentile(lat, 50)), "p90_ms": float(np.percentile(lat, 90)),
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+            "by_request_frames": buckets,
+            "p99_ms_1k_frame_requests": buckets.get("900-1100", {}).get("p99_ms"),
             "throughput_pairs_s": frames * n_res / span_s,
             "offered_pairs_s": frames * n_res / args.serving_seconds,
             "gpu_busy_frac": st["busy_ms"] / (span_s * 1e3),
@@ -397,9 +410,10 @@ def main():
                     help="each served batch's contraction takes 1/slots of the CUs (1) or all (0)")
     ap.add_argument("--serve-graphs", type=int, default=1,
                     help="replay each served batch's kernels as a hipGraph (1) or launch them (0)")
-    ap.add_argument("--mixed-seconds", type=float, default=0.0,
+    ap.add_argument("--mixed-seconds", type=float, default=20.0,
                     help="run the three-set mixed serving workload and its graph check for this "
-                         "many seconds (0 = skip; the CPU side of the graph check takes minutes)")
+                         "many seconds (0 = skip; the CPU side of the graph check, the FP64 "
+                         "oracle kNN of 30,000 frames, takes about 40 s on 16 threads)")
     ap.add_argument("--mixed-rate", type=float, default=100.0, help="mixed workload arrivals per second")
     ap.add_argument("--cpu-rows", type=int, default=128, help="fit rows of the CPU baseline sample (0 = skip)")
     ap.add_argument("--check-rows", type=int, default=8, help="rows checked against the FP64 oracle")
