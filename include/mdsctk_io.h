@@ -30,6 +30,24 @@ int mdsctk_xtc_info(const char* path, int64_t* n_frames, int32_t* natoms, float*
 int mdsctk_xtc_read(const char* path, float* xyz, int64_t max_frames, int32_t natoms,
                     int64_t* n_read, int32_t* steps, float* times, float* boxes, int nthreads);
 
+/* Decode a whole XTC file on HIP device `device`: the file is read into
+ * pinned host chunks, each uploaded while the next is read, the frame
+ * headers are walked as the chunks arrive, and the frames are decoded on
+ * the device, one lane per frame (same values as mdsctk_xtc_read).
+ * *d_xyz receives a device buffer [n][natoms][3] (nm) that the caller
+ * releases with mdsctk_xtc_free_device (NULL for an empty file), *n_read the
+ * frame count.  natoms must match the file (MDSCTK_ERR_NATOMS otherwise, as
+ * knn_rms.cpp:158-178).  Replaces the host-side read loop of
+ * knn_rms.cpp:186-206 for callers that compute on the device. */
+int mdsctk_xtc_read_device(const char* path, int32_t natoms, int device, float** d_xyz,
+                           int64_t* n_read);
+
+/* Atom count of the first frame of an XTC file (reads its header only). */
+int mdsctk_xtc_natoms(const char* path, int32_t* natoms);
+
+/* Release a buffer returned by mdsctk_xtc_read_device. */
+int mdsctk_xtc_free_device(float* d_xyz);
+
 /* Write frames as XTC with the given precision (e.g. 1000 = 1e-3 nm).
  * steps/times/boxes may be NULL (step = frame index, time = 0, zero box). */
 int mdsctk_xtc_write(const char* path, const float* xyz, int64_t n_frames, int32_t natoms,

@@ -79,13 +79,18 @@ extern "C" {
                                         batches follow one another */
 #define MDSCTK_OPT_SELF_SYMMETRIC 8 /* 1 (default): a self query (mdsctk_knn_query with
                                        xyz NULL) scores each unordered pair once: the
-                                       contraction covers the upper block triangle and
-                                       writes an n x n FP32 score matrix (4 n^2 bytes, plus
-                                       16 KiB per frame of candidate lists), from which each
-                                       row's candidates are selected.  Used only when that
-                                       fits the free device memory and the HBM budget;
-                                       outputs are identical either way.  0: the candidate-
-                                       list path */
+                                       contraction covers the upper block triangle, and a
+                                       pair contracted one way round also joins the other
+                                       frame's mirrored list (64 KiB per frame by default,
+                                       MDSCTK_OPT_MIRROR_CAP, plus the
+                                       candidate lists).  No score matrix is written.  Used
+                                       when k+1 <= 896 and the lists fit the free device
+                                       memory and the HBM budget; outputs are identical
+                                       either way.  0: the candidate-list path */
+#define MDSCTK_OPT_MIRROR_CAP 9     /* entries of each frame's mirrored list in the symmetric
+                                       self query (default 8192, 8 bytes each).  A frame
+                                       whose list overflows is recomputed through the
+                                       candidate lists, with the same outputs */
 
 typedef struct mdsctk_knn_ctx mdsctk_knn_ctx;
 
@@ -105,6 +110,12 @@ typedef struct mdsctk_knn_stats {
   double eps_rel;          /* candidate window relative to (G_q + max G_ref) */
   int64_t n_pairs_computed; /* pairs the contraction scored: n_pairs, or about half of
                                them for a symmetric self query */
+  double ms_landmark;       /* symmetric self query: its landmark pass (thresholds) */
+  int64_t n_pairs_landmark; /* pairs the landmark pass scored */
+  int64_t n_mirrored;       /* scores appended to frames' mirrored lists */
+  int64_t n_mirrored_max;   /* most appended to one frame's mirrored list */
+  int64_t n_relisted_rows;  /* rows recomputed through the candidate lists (a mirrored
+                               list overflowed) */
 } mdsctk_knn_stats;
 
 /* Create a context on HIP device `device` with the given MDSCTK_PREC_*.

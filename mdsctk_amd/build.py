@@ -37,7 +37,8 @@ DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
           f"-I{os.path.join(ROOT, 'include')}"]
 
-LIB_SOURCES = ["knn_kernels.hip", "api.cpp", "serve.cpp", "xtc.cpp", "topology.cpp", "spectral.cpp"]
+LIB_SOURCES = ["knn_kernels.hip", "xtc_kernels.hip", "api.cpp", "serve.cpp", "xtc.cpp",
+               "xtc_device.cpp", "topology.cpp", "spectral.cpp"]
 # executables: (name, main source)
 TOOLS = [("knn_rms", "knn_rms_main.cpp"), ("make_sysparse", "make_sysparse_main.cpp"),
          ("decomp_sparse", "decomp_sparse_main.cpp"), ("make_gesparse", "make_gesparse_main.cpp"),

@@ -265,7 +265,7 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
   server_stop(c);
   HIP_TRY(c, hipSetDevice(c->device));
   free_frames(c, c->ref);
-  release_score_matrix(c);
+  release_mirror_lists(c);
   dfree(c, c->data_shift);
   c->data_shift = nullptr;
   c->has_ref = false;
@@ -328,7 +328,7 @@ int set_data_reference_impl(mdsctk_knn_ctx* c, const double* x, int64_t n_ref, i
   server_stop(c);
   HIP_TRY(c, hipSetDevice(c->device));
   free_frames(c, c->ref);
-  release_score_matrix(c);
+  release_mirror_lists(c);
   dfree(c, c->data_shift);
   c->data_shift = nullptr;
   c->has_ref = false;
@@ -375,7 +375,7 @@ int slot_init(mdsctk_knn_ctx* c, Slot& s) {
   return 0;
 }
 
-void release_score_matrix(mdsctk_knn_ctx* c) {
+void release_mirror_lists(mdsctk_knn_ctx* c) {
   Slot& s = c->main;
   if (!s.ws_sym) return;
   if (s.st) (void)hipStreamSynchronize(s.st);
@@ -393,6 +393,7 @@ void slot_release(mdsctk_knn_ctx* c, Slot& s) {
   dfree(c, s.ws_out);
   dfree(c, s.ws_rows);
   dfree(c, s.ws_sym);
+  dfree(c, s.ws_relist);
   for (auto& e : s.ev)
     if (e) (void)hipEventDestroy(e);
   if (s.st) (void)hipStreamDestroy(s.st);
@@ -508,26 +509,51 @@ size_t fused_list_entries(const mdsctk_knn_ctx* c, int64_t n_fit, int k1) {
   return (size_t)n_fit * phases * 4 * list_capacity_for(k1);
 }
 
-int plan_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, FusedPlan* p) {
+int plan_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, FusedPlan* p, Carve* cv) {
   const DevFrames& fit = s.fit;
   const int64_t n_fit = fit.n;
   const int cap = list_capacity_for(q.k1);
   p->grid = contract_grid(c);
   p->phases = fused_phases(c, fit.n_tiles);
+  auto ent_bytes_for = [&](int phases) { return (size_t)n_fit * phases * 4 * cap * sizeof(uint2); };
+  auto small_bytes_for = [&](int phases) {
+    return (size_t)n_fit * phases * 4 * (sizeof(int) + sizeof(float)) +
+           (size_t)n_fit * (sizeof(unsigned) + sizeof(float) + sizeof(int)) + 256;
+  };
+  void* ent = nullptr;
+  void* small_p = nullptr;
+  if (cv) {
+    // Fewer phases mean fewer lists per query (and fewer, longer work
+    // items); one exact row's scratch stays in reserve for the fallback.
+    const size_t reserve = Carve::round((size_t)c->ref.n * sizeof(double) + sizeof(int));
+    auto need = [&](int ph) {
+      return Carve::round(ent_bytes_for(ph)) + Carve::round(small_bytes_for(ph)) + reserve;
+    };
+    while (p->phases > 1 && need(p->phases) > cv->left()) p->phases /= 2;
+    if (need(p->phases) > cv->left())
+      return fail(c, MDSCTK_ERR_ARG,
+                  "query (fixed workspace): %lld query frames with k+1=%d need %zu bytes of workspace, "
+                  "more than the %zu left of the %zu-byte fixed region",
+                  (long long)n_fit, q.k1, need(p->phases), cv->left(), cv->size);
+    ent = cv->take(ent_bytes_for(p->phases));
+    small_p = cv->take(small_bytes_for(p->phases));
+  }
   Lists& L = p->L;
   L = Lists();
   L.nl = p->phases * 4;
   L.cap = cap;
+  if (!cv) {
+    int rc = ensure(c, &s.ws_lists, &s.ws_lists_bytes, ent_bytes_for(p->phases),
+                    "candidate lists");
+    if (rc) return rc;
+    rc = ensure(c, &s.ws_small, &s.ws_small_bytes, small_bytes_for(p->phases), "list headers");
+    if (rc) return rc;
+    ent = s.ws_lists;
+    small_p = s.ws_small;
+  }
   const size_t n_hdr = (size_t)n_fit * L.nl;
-  const size_t ent_bytes = n_hdr * cap * sizeof(uint2);
-  int rc = ensure(c, &s.ws_lists, &s.ws_lists_bytes, ent_bytes, "candidate lists");
-  if (rc) return rc;
-  L.ent = (uint2*)s.ws_lists;
-  const size_t small = n_hdr * (sizeof(int) + sizeof(float)) +
-                       (size_t)n_fit * (sizeof(unsigned) + sizeof(float) + sizeof(int)) + 256;
-  rc = ensure(c, &s.ws_small, &s.ws_small_bytes, small, "list headers");
-  if (rc) return rc;
-  char* sp = (char*)s.ws_small;
+  L.ent = (uint2*)ent;
+  char* sp = (char*)small_p;
   L.cnt = (int*)sp; sp += n_hdr * sizeof(int);
   L.ovf = (float*)sp; sp += n_hdr * sizeof(float);
   L.thr = (unsigned*)sp; sp += n_fit * sizeof(unsigned);
@@ -555,9 +581,9 @@ int enqueue_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, const FusedPl
 }
 
 int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
-                 FusedInfo* info) {
+                 FusedInfo* info, Carve* cv) {
   FusedPlan p;
-  int rc = plan_fused(c, s, q, &p);
+  int rc = plan_fused(c, s, q, &p, cv);
   if (rc) return rc;
   if ((rc = enqueue_fused(c, s, q, p, d_dist, d_idx, true))) return rc;
   info->phases = p.phases;
@@ -573,67 +599,90 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
   const DevFrames& fit = s.fit;
   const int64_t n = fit.n;
   if (n != c->ref.n || fit.n_tiles != c->ref.n_tiles || n < 1) return 1;
-  const int64_t ld = (n + 3) & ~int64_t(3);
-  const size_t sym_bytes = (size_t)n * (size_t)ld * sizeof(float);
-  const int cap = kSymListCap;
-  const size_t ent_bytes = (size_t)n * cap * sizeof(uint2);
-  // Room: the free device memory plus what the two buffers already hold,
-  // less a reserve for everything else; and the context's budget.
+  const int cap = list_capacity_for(q.k1);
+  if (cap == 0) return 1;
+  const int ref_pairs = c->ref.n_tiles / kRefTilesPerWG;
+  // Each fit frame sees about half of its pairs, so half the list form's
+  // phases (longer lists per phase, tighter list thresholds).
+  int phases = fused_phases(c, fit.n_tiles);
+  if (c->forced_phases <= 0) phases = std::max(1, std::min(ref_pairs, std::max(4, phases / 2)));
+  const int nl = phases * 4;
+  // Landmark pass: every lstride-th reference tile pair, about 4 (k+1)
+  // frames or more (stride at most kLandmarkStride, 1/64 of the set).
+  const int lstride =
+      std::max(1, std::min<int>(kLandmarkStride, (int)((int64_t)ref_pairs * 2 * kTile / (4 * q.k1))));
+  const int lm_pairs = (ref_pairs + lstride - 1) / lstride;
+  const int nlab = lm_pairs * 2 * kTile;
+  const size_t ent_bytes = (size_t)n * nl * cap * sizeof(uint2);
+  const size_t mir_bytes = (size_t)n * c->mirror_cap * sizeof(uint2);
+  const size_t small = (size_t)n * nl * (sizeof(int) + sizeof(float)) +
+                       (size_t)n * 7 * sizeof(int) + (size_t)nlab * sizeof(int) + 1024;
+  // Room: the free device memory plus what the buffers already hold, less a
+  // reserve for everything else; and the context's budget.
   size_t free_b = 0, total_b = 0;
   HIP_TRY(c, hipMemGetInfo(&free_b, &total_b));
-  const size_t have = s.ws_sym_bytes + s.ws_lists_bytes;
+  const size_t have = s.ws_sym_bytes + s.ws_lists_bytes + s.ws_small_bytes;
+  const size_t need = ent_bytes + mir_bytes + small;
   const size_t reserve = size_t(2) << 30;
-  if (sym_bytes + ent_bytes + reserve > free_b + have) return 1;
+  if (need + reserve > free_b + have) return 1;
   {
     std::lock_guard<std::mutex> g(c->mem_mu);
-    if (c->budget && c->held - std::min(c->held, have) + sym_bytes + ent_bytes > c->budget)
-      return 1;
+    if (c->budget && c->held - std::min(c->held, have) + need > c->budget) return 1;
   }
-  const size_t small = (size_t)n * (sizeof(int) + sizeof(float)) +
-                       (size_t)n * (sizeof(float) + sizeof(int)) +
-                       (size_t)n * 2 * sizeof(int) + 512;
-  int rc = ensure(c, &s.ws_sym, &s.ws_sym_bytes, sym_bytes, "score matrix");
+  int rc = ensure(c, &s.ws_sym, &s.ws_sym_bytes, mir_bytes, "mirrored lists");
   if (!rc) rc = ensure(c, &s.ws_lists, &s.ws_lists_bytes, ent_bytes, "candidate lists");
   if (!rc) rc = ensure(c, &s.ws_small, &s.ws_small_bytes, small, "list headers");
-  if (rc == MDSCTK_ERR_NOMEM) {  // the list path instead, without the matrix
-    dfree(c, s.ws_sym);
-    s.ws_sym = nullptr;
-    s.ws_sym_bytes = 0;
+  if (rc == MDSCTK_ERR_NOMEM) {  // the list path instead, without the mirrored lists
+    release_mirror_lists(c);
     c->err.clear();
     return 1;
   }
   if (rc) return rc;
   Lists L;
-  L.nl = 1;
+  L.nl = nl;
   L.cap = cap;
   L.ent = (uint2*)s.ws_lists;
   char* sp = (char*)s.ws_small;
   double* gmax_q = (double*)sp; sp += 256;
-  int* counts = (int*)sp; sp += 256;
-  L.cnt = (int*)sp; sp += n * sizeof(int);
-  L.ovf = (float*)sp; sp += n * sizeof(float);
+  unsigned long long* msum = (unsigned long long*)sp; sp += 256;
+  L.cnt = (int*)sp; sp += (size_t)n * nl * sizeof(int);
+  L.ovf = (float*)sp; sp += (size_t)n * nl * sizeof(float);
+  L.thr = (unsigned*)sp; sp += n * sizeof(unsigned);
   float* eps2 = (float*)sp; sp += n * sizeof(float);
   int* label = (int*)sp; sp += n * sizeof(int);
   int* order = (int*)sp; sp += n * sizeof(int);
-  int* d_fallback = (int*)sp;
+  int* d_fallback = (int*)sp; sp += n * sizeof(int);
+  int* counts = (int*)sp; sp += (size_t)nlab * sizeof(int);
+  Mirror M;
+  M.ent = (uint2*)s.ws_sym;
+  M.cap = c->mirror_cap;
+  M.cnt = (int*)sp; sp += n * sizeof(int);
+  M.ovf = (float*)sp;
   const double kappa = (c->w.metric == kMetricRMSD ? kappa_for(c->prec, c->ref.natoms)
                                                    : kappa_data(c->prec, c->ref.natoms)) *
                        c->eps_scale;
   const int grid = contract_grid(c);
-  const int phases = fused_phases(c, fit.n_tiles);
-  float* sym = (float*)s.ws_sym;
   HIP_TRY(c, launch_eps_sym(fit, c->ref, kappa, gmax_q, eps2, s.n_cand, s.st));
   HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
-  HIP_TRY(c, launch_contract(c->ref, fit, L, eps2, q.k1, phases, q.nofit, nullptr, grid, s.st,
-                             sym, ld));
+  // landmark pass into the first 4 lists' worth of the list workspace
+  Lists LL = L;
+  LL.nl = 4;
+  HIP_TRY(c, launch_init_lists(LL, n, s.st));
+  HIP_TRY(c, launch_contract(c->ref, fit, LL, eps2, q.k1, lstride, q.nofit, nullptr, grid, s.st,
+                             nullptr, 1));
+  HIP_TRY(c, launch_landmark_cap(LL, n, q.k1, eps2, lstride, L.thr, label, s.st));
+  HIP_TRY(c, launch_init_lists(L, n, s.st, false));
+  HIP_TRY(c, launch_mirror_init(M, n, s.st));
+  HIP_TRY(c, hipEventRecord(s.ev[6], s.st));
+  HIP_TRY(c, launch_contract(c->ref, fit, L, eps2, q.k1, phases, q.nofit, nullptr, grid, s.st, &M));
   HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
-  HIP_TRY(c, launch_sym_select(sym, ld, n, eps2, q.k1, L, s.st));
-  // merge rows grouped by landmark (it pays where candidate frames no
-  // longer fit the caches; launch_sym_order needs n >= 64)
+  HIP_TRY(c, launch_mirror_sum(M, n, msum, s.st));
+  // merge rows grouped by nearest landmark (it pays where candidate frames
+  // no longer fit the caches)
   const bool ordered = n >= 512;
-  if (ordered) HIP_TRY(c, launch_sym_order(sym, ld, n, label, counts, order, s.st));
+  if (ordered) HIP_TRY(c, launch_row_order(label, n, nlab, counts, order, s.st));
   HIP_TRY(c, launch_merge(c->ref, fit, c->w, L, eps2, q.k1, q.k_out, q.rank_off, q.nofit, d_dist,
-                          d_idx, d_fallback, s.n_cand, ordered ? order : nullptr, s.st));
+                          d_idx, d_fallback, s.n_cand, ordered ? order : nullptr, s.st, &M));
   HIP_TRY(c, hipEventRecord(s.ev[4], s.st));
   // pairs of the upper block triangle: fit block fb against frames from
   // its own first frame on
@@ -645,21 +694,67 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
   info->kappa = kappa;
   info->d_fallback = d_fallback;
   info->pairs_computed = pairs;
+  info->symmetric = true;
+  info->pairs_landmark = n * std::min<int64_t>(n, (int64_t)lm_pairs * 2 * kTile);
+  info->d_mirror_sum = msum;
+  return 0;
+}
+
+int relist_rows(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, std::vector<int>& rows,
+                double* d_dist, int* d_idx) {
+  const int nr = (int)rows.size();
+  if (nr == 0) return 0;
+  const size_t out_elems = (size_t)nr * q.k_out;
+  int rc = ensure(c, &s.ws_relist, &s.ws_relist_bytes,
+                  Carve::round(nr * sizeof(int)) + out_elems * (sizeof(double) + sizeof(int)),
+                  "relisted rows");
+  if (rc) return rc;
+  int* d_rows = (int*)s.ws_relist;
+  double* t_dist = (double*)((char*)s.ws_relist + Carve::round(nr * sizeof(int)));
+  int* t_idx = (int*)(t_dist + out_elems);
+  HIP_TRY(c, hipMemcpyAsync(d_rows, rows.data(), nr * sizeof(int), hipMemcpyHostToDevice, s.st));
+  if ((rc = slot_frames(c, s, nr))) return rc;
+  HIP_TRY(c, launch_gather_frames(c->ref.xyz, d_rows, nr, c->ref.natoms, s.fit.xyz, s.st));
+  if ((rc = prep_slot(c, s))) return rc;
+  FusedInfo info;
+  if ((rc = launch_fused(c, s, q, t_dist, t_idx, &info))) return rc;
+  HIP_TRY(c, launch_scatter_rows(t_dist, t_idx, d_rows, nr, q.k_out, d_dist, d_idx, s.st));
+  std::vector<int> fb(nr);
+  HIP_TRY(c, hipMemcpyAsync(fb.data(), info.d_fallback, nr * sizeof(int), hipMemcpyDeviceToHost,
+                            s.st));
+  HIP_TRY(c, hipStreamSynchronize(s.st));
+  std::vector<int> left;
+  for (int i = 0; i < nr; ++i)
+    if (fb[i]) left.push_back(rows[i]);
+  rows.swap(left);
   return 0;
 }
 
 // Exact FP64 path for the listed fit rows: scores of whole rows in batches,
 // then (s, index) selection.  rows == nullptr means rows [0, nrows).
 int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const int* h_rows,
-                      int nrows, const QueryShape& q, double* d_dist, int* d_idx) {
+                      int nrows, const QueryShape& q, double* d_dist, int* d_idx, Carve* cv) {
   const int64_t n_ref = c->ref.n;
   const size_t row_bytes = (size_t)n_ref * sizeof(double);
   int batch = (int)std::max<int64_t>(1, std::min<int64_t>(nrows, (int64_t)(1ull << 30) / row_bytes));
-  int rc = ensure(c, &s.ws_rows, &s.ws_rows_bytes, row_bytes * batch + sizeof(int) * batch,
-                  "exact-row scratch");
-  if (rc) return rc;
-  double* srow = (double*)s.ws_rows;
-  int* drows = (int*)((char*)s.ws_rows + row_bytes * batch);
+  void* ws = nullptr;
+  if (cv) {
+    const size_t room = cv->left() > 256 ? cv->left() - 256 : 0;
+    batch = (int)std::min<int64_t>(batch, (int64_t)(room / (row_bytes + sizeof(int))));
+    if (batch < 1)
+      return fail(c, MDSCTK_ERR_ARG,
+                  "query (fixed workspace): one exact row of %lld references does not fit the %zu bytes "
+                  "left of the fixed region",
+                  (long long)n_ref, cv->left());
+    ws = cv->take(row_bytes * batch + sizeof(int) * batch);
+  } else {
+    int rc = ensure(c, &s.ws_rows, &s.ws_rows_bytes, row_bytes * batch + sizeof(int) * batch,
+                    "exact-row scratch");
+    if (rc) return rc;
+    ws = s.ws_rows;
+  }
+  double* srow = (double*)ws;
+  int* drows = (int*)((char*)ws + row_bytes * batch);
   std::vector<int> ids(batch);
   for (int b0 = 0; b0 < nrows; b0 += batch) {
     const int nb = std::min(batch, nrows - b0);
@@ -679,7 +774,7 @@ namespace {
 // FP64 vectors; nullptr = the resident set itself.
 int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, int64_t n_fit,
                int32_t k, uint32_t flags, double* out_dist, int32_t* out_idx,
-               bool out_on_device) {
+               bool out_on_device, Carve* cv = nullptr) {
   if (!c) return MDSCTK_ERR_ARG;
   if (!c->has_ref) return fail(c, MDSCTK_ERR_STATE, "query: no reference set");
   if (data != (c->w.metric != kMetricRMSD))
@@ -701,18 +796,53 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
     return q.k_out;
   }
   HIP_TRY(c, hipSetDevice(c->device));
+  // A query that is not a self query does not need the self query's
+  // mirrored lists; their memory goes back to the lists it does need.
+  if (xyz && !cv) release_mirror_lists(c);
   Slot& s = c->main;
   mdsctk_knn_stats S = {};
   S.n_pairs = n_fit * c->ref.n;
   S.n_pairs_computed = S.n_pairs;
   HIP_TRY(c, hipEventRecord(s.ev[0], s.st));
 
-  // Stage (or alias) the query frames and run their prep pass.
+  // Stage (or alias) the query frames and run their prep pass.  Without
+  // allocation (cv) the query frames are read where the caller holds them
+  // and their derived arrays are carved from the fixed region; the slot's
+  // own frame set is left untouched.
   const DevFrames* fitp = &c->ref;
-  if (xyz) {
+  Slot cs;
+  if (cv) {
+    cs = s;
+    cs.fit = DevFrames();
+    cs.n_cand = (int*)cv->take(sizeof(int));
+    if (xyz) {
+      DevFrames& f = cs.fit;
+      f.n = n_fit;
+      f.natoms = c->ref.natoms;
+      f.prec = c->ref.prec;
+      f.n_tiles = std::max(kTileAlign, padded_tiles(n_fit));
+      f.cap_tiles = f.n_tiles;
+      f.n_groups = c->ref.n_groups;
+      f.xyz = (float*)xyz;
+      f.owns_xyz = false;
+      const size_t rows = (size_t)f.n_tiles * kTile;
+      f.cen = (double*)cv->take(rows * 3 * sizeof(double));
+      f.G = (double*)cv->take(rows * sizeof(double));
+      f.Gt = (double*)cv->take(rows * sizeof(double));
+      if (c->prec != kExact) f.op = (uint4*)cv->take(operand_bytes(f.prec, f.n_tiles, f.n_groups));
+      if (!cs.n_cand || !f.cen || !f.G || !f.Gt || (c->prec != kExact && !f.op))
+        return fail(c, MDSCTK_ERR_ARG,
+                    "query (fixed workspace): %lld query frames of %d atoms do not fit the %zu-byte "
+                    "fixed region",
+                    (long long)n_fit, f.natoms, cv->size);
+      if ((rc = prep_frames(c, f, cs.st, false))) return rc;
+      fitp = &cs.fit;
+    }
+  } else if (xyz) {
     if ((rc = stage_input(c, s, xyz, on_device, n_fit))) return rc;
     fitp = &s.fit;
   }
+  Slot& ws = cv ? cs : s;
   HIP_TRY(c, hipEventRecord(s.ev[1], s.st));
 
   const size_t out_elems = (size_t)n_fit * q.k_out;
@@ -728,19 +858,17 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
 
   float ms_contract = 0, ms_merge = 0, ms_fallback = 0;
   // The fused pipeline runs on the slot's frames; a self query points the
-  // slot at the resident set for its duration.  A self query whose k+1 the
-  // candidate lists cannot hold still takes the symmetric form when it
-  // fits (its select list and merge hold up to 2048 candidates).
-  const bool sym_wide = !q.full && !q.fused && !xyz && c->self_sym && c->prec != kExact &&
-                        q.k1 <= kSymMaxK1;
+  // slot at the resident set for its duration and, when it fits, scores
+  // each unordered pair once (the symmetric form).  Without allocation a
+  // self query takes the candidate lists.
   int fused_rc = 1;  // 1: the fused pipeline did not run
   FusedInfo info;
-  if (!q.full && (q.fused || sym_wide)) {
-    DevFrames saved = s.fit;
-    if (!xyz) s.fit = query_view(c->ref);
-    if (!xyz && c->self_sym) fused_rc = launch_fused_sym(c, s, q, d_dist, d_idx, &info);
-    if (fused_rc == 1 && q.fused) fused_rc = launch_fused(c, s, q, d_dist, d_idx, &info);
-    if (!xyz) s.fit = saved;
+  if (!q.full && q.fused) {
+    DevFrames saved = ws.fit;
+    if (!xyz) ws.fit = query_view(c->ref);
+    if (!xyz && c->self_sym && !cv) fused_rc = launch_fused_sym(c, ws, q, d_dist, d_idx, &info);
+    if (fused_rc == 1) fused_rc = launch_fused(c, ws, q, d_dist, d_idx, &info, cv);
+    if (!xyz) ws.fit = saved;
     if (fused_rc < 0) return fused_rc;
   }
   if (q.full) {
@@ -752,7 +880,7 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
     ms_contract = elapsed(s.ev[1], s.ev[2]);
   } else if (fused_rc == 1) {
     HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
-    rc = exact_rows_select(c, s, *fitp, nullptr, (int)n_fit, q, d_dist, d_idx);
+    rc = exact_rows_select(c, ws, *fitp, nullptr, (int)n_fit, q, d_dist, d_idx, cv);
     if (rc) return rc;
     HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
     HIP_TRY(c, hipEventSynchronize(s.ev[3]));
@@ -761,24 +889,44 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
   } else {
     std::vector<int> fb(n_fit);
     int ncand = 0;
+    unsigned long long msum[2] = {0, 0};
     HIP_TRY(c, hipMemcpyAsync(fb.data(), info.d_fallback, n_fit * sizeof(int),
                               hipMemcpyDeviceToHost, s.st));
-    HIP_TRY(c, hipMemcpyAsync(&ncand, s.n_cand, sizeof(int), hipMemcpyDeviceToHost, s.st));
+    HIP_TRY(c, hipMemcpyAsync(&ncand, ws.n_cand, sizeof(int), hipMemcpyDeviceToHost, s.st));
+    if (info.d_mirror_sum)
+      HIP_TRY(c, hipMemcpyAsync(msum, info.d_mirror_sum, sizeof msum, hipMemcpyDeviceToHost, s.st));
     HIP_TRY(c, hipStreamSynchronize(s.st));
-    ms_contract = elapsed(s.ev[2], s.ev[3]);
+    if (info.symmetric) {
+      S.ms_landmark = elapsed(s.ev[2], s.ev[6]);
+      ms_contract = elapsed(s.ev[6], s.ev[3]);
+    } else {
+      ms_contract = elapsed(s.ev[2], s.ev[3]);
+    }
     ms_merge = elapsed(s.ev[3], s.ev[4]);
     S.n_candidates = ncand;
     S.ref_splits = info.phases;
     S.list_capacity = info.cap;
     S.eps_rel = info.kappa;
     S.n_pairs_computed = info.pairs_computed;
+    S.n_pairs_landmark = info.pairs_landmark;
+    S.n_mirrored = (int64_t)msum[0];
+    S.n_mirrored_max = (int64_t)msum[1];
     std::vector<int> rows;
     for (int64_t i = 0; i < n_fit; ++i)
       if (fb[i]) rows.push_back((int)i);
     if (!rows.empty()) {
       HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
-      rc = exact_rows_select(c, s, *fitp, rows.data(), (int)rows.size(), q, d_dist, d_idx);
-      if (rc) return rc;
+      // rows of a symmetric self query whose mirrored list overflowed go
+      // through the candidate lists first, the rest (and any they flag) to
+      // the exact path
+      if (info.symmetric) {
+        S.n_relisted_rows = (int64_t)rows.size();
+        if ((rc = relist_rows(c, s, q, rows, d_dist, d_idx))) return rc;
+      }
+      if (!rows.empty()) {
+        rc = exact_rows_select(c, ws, *fitp, rows.data(), (int)rows.size(), q, d_dist, d_idx, cv);
+        if (rc) return rc;
+      }
       HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
       HIP_TRY(c, hipEventSynchronize(s.ev[3]));
       ms_fallback = elapsed(s.ev[2], s.ev[3]);
@@ -793,14 +941,14 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
   }
   HIP_TRY(c, hipEventRecord(s.ev[5], s.st));
   HIP_TRY(c, hipEventSynchronize(s.ev[5]));
-  // While serving runs, a self query does not keep its score matrix: the
+  // While serving runs, a self query does not keep its mirrored lists: the
   // memory goes back to the serving slots' workspace.
   bool serving;
   {
     std::lock_guard<std::mutex> g(c->server_mu);
     serving = c->server != nullptr;
   }
-  if (serving) release_score_matrix(c);
+  if (serving && !cv) release_mirror_lists(c);
   S.ms_prep = elapsed(s.ev[0], s.ev[1]);
   S.ms_contract = ms_contract;
   S.ms_merge = ms_merge;
@@ -931,6 +1079,12 @@ int mdsctk_knn_set_option(mdsctk_knn_ctx* c, int option, double value) {
       return 0;
     case MDSCTK_OPT_SELF_SYMMETRIC:
       c->self_sym = value != 0;
+      return 0;
+    case MDSCTK_OPT_MIRROR_CAP:
+      if (!(value >= 1 && value <= (1 << 20)))
+        return fail(c, MDSCTK_ERR_ARG, "mirrored list capacity must be 1..1048576");
+      release_mirror_lists(c);
+      c->mirror_cap = (int)value;
       return 0;
     case MDSCTK_OPT_SERVE_GRAPHS:
       if (c->server) return fail(c, MDSCTK_ERR_STATE, "serving already started");

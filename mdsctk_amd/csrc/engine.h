@@ -36,9 +36,14 @@ struct Slot {
   size_t ws_out_bytes = 0;
   void* ws_rows = nullptr;
   size_t ws_rows_bytes = 0;
-  void* ws_sym = nullptr;  // score matrix of a symmetric self query
+  void* ws_sym = nullptr;  // mirrored lists of a symmetric self query
   size_t ws_sym_bytes = 0;
-  hipEvent_t ev[6] = {};
+  void* ws_relist = nullptr;  // row ids and outputs of rows recomputed through the lists
+  size_t ws_relist_bytes = 0;
+  // ev[0] start, ev[1] query prepared, ev[2] before the contraction, ev[3]
+  // after it, ev[4] after the merge, ev[5] end; ev[6] after a symmetric self
+  // query's landmark pass (so ev[2]..ev[6] is that pass).
+  hipEvent_t ev[8] = {};
 };
 
 // Shape of one query: output width, ranks kept, and the route it takes.
@@ -69,7 +74,29 @@ struct FusedInfo {
   int cap = 0;
   double kappa = 0;
   int* d_fallback = nullptr;  // [n_fit] device flags (in the slot's workspace)
-  int64_t pairs_computed = 0;  // pairs the contraction scored
+  int64_t pairs_computed = 0;  // pairs the main contraction scored
+  // symmetric self query only
+  bool symmetric = false;
+  int64_t pairs_landmark = 0;  // pairs of its landmark pass (0: none)
+  unsigned long long* d_mirror_sum = nullptr;  // [2]: mirrored appends, most per frame
+};
+
+// Pieces of caller-provided device memory handed out in order, 256-byte
+// aligned: the allocation-free query carves its whole workspace from the
+// per-GPU fixed region this way.
+struct Carve {
+  char* base = nullptr;
+  size_t size = 0;
+  size_t used = 0;
+  static size_t round(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+  size_t left() const { return size - used; }
+  void* take(size_t bytes) {
+    const size_t b = round(bytes ? bytes : 1);
+    if (!base || b > left()) return nullptr;
+    void* p = base + used;
+    used += b;
+    return p;
+  }
 };
 
 class Server;  // serve.cpp
@@ -92,7 +119,8 @@ struct mdsctk_knn_ctx {
   int64_t serve_max_frames = 8192;
   bool serve_graphs = true;  // replay each served batch's kernels as a hipGraph
   bool serve_share_cus = true;  // each batch's contraction takes 1/serve_slots of the CUs
-  bool self_sym = true;  // self queries through the score matrix when it fits
+  bool self_sym = true;  // self queries score each unordered pair once when the lists fit
+  int mirror_cap = mdsctk::kMirrorCap;  // entries of each frame's mirrored list
 
   mdsctk::DevWeights w;  // masses, or unit masses plus the metric for data vectors
   mdsctk::DevFrames ref;
@@ -119,11 +147,12 @@ int ensure(mdsctk_knn_ctx* c, void** p, size_t* have, size_t need, const char* w
 
 int slot_init(mdsctk_knn_ctx* c, Slot& s);
 void slot_release(mdsctk_knn_ctx* c, Slot& s);
-// Free the self query's score matrix, which the context otherwise keeps
-// for the next self query.  Called by the synchronous entry points only (a
-// new reference set; the end of a self query while serving runs), so it
-// never races a query that uses the matrix.
-void release_score_matrix(mdsctk_knn_ctx* c);
+// Free the symmetric self query's mirrored lists, which the context
+// otherwise keeps for the next self query.  Called by the synchronous entry
+// points only (a new reference set; a query that is not a self query; the
+// end of a self query while serving runs), so it never races a query that
+// uses them.
+void release_mirror_lists(mdsctk_knn_ctx* c);
 
 // Validate k/flags against the resident set and fill `q`.
 int resolve_shape(mdsctk_knn_ctx* c, int32_t k, uint32_t flags, QueryShape* q);
@@ -144,23 +173,37 @@ size_t fused_list_entries(const mdsctk_knn_ctx* c, int64_t n_fit, int k1);
 // enqueue_fused only enqueues kernels on the slot's stream (with timing,
 // also events ev[2..4]), outputs to d_dist/d_idx; launch_fused does both
 // with timing.
-int plan_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, FusedPlan* p);
+// With `cv`, plan_fused takes the workspace from it instead of the slot's
+// buffers (allocating nothing), lowering the reference phases until the
+// candidate lists fit; it fails with MDSCTK_ERR_ARG when even one phase does
+// not.
+int plan_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, FusedPlan* p,
+               Carve* cv = nullptr);
 int enqueue_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, const FusedPlan& p,
                   double* d_dist, int* d_idx, bool timing);
 int launch_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
-                 FusedInfo* info);
+                 FusedInfo* info, Carve* cv = nullptr);
 // A self query (the slot's frames are the resident set's query view)
-// through the n x n FP32 score matrix: the contraction scores the upper block
-// triangle only, each row's candidates are selected from the matrix, and
-// the merge re-scores them as for the lists, with timing.  Returns 1 with
-// nothing enqueued when the matrix and its lists do not fit the free device
-// memory (or the context's budget).
+// scoring each unordered pair once: a landmark pass bounds every row's
+// threshold, the contraction scores the upper block triangle into each fit
+// frame's lists and each reference frame's mirrored list, and the merge
+// re-scores both as for the lists, with timing.  No score matrix is
+// written.  Returns 1 with nothing enqueued when the lists do not fit the
+// free device memory (or the context's budget), or k+1 exceeds what the
+// lists hold.
 int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_dist, int* d_idx,
                      FusedInfo* info);
+// Rows of a symmetric self query that it could not certify (a mirrored
+// list overflowed) again through the candidate lists, their frames gathered
+// from the resident set; `rows` keeps those still uncertified.
+int relist_rows(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, std::vector<int>& rows,
+                double* d_dist, int* d_idx);
 // Exact FP64 rows for the listed fit rows (nullptr = rows [0,nrows)) of
-// the slot's query frames, synchronously on the slot's stream.
+// the slot's query frames, synchronously on the slot's stream.  Its row
+// scratch comes from `cv` when given (else the slot's grow-only buffer).
 int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const int* h_rows,
-                      int nrows, const QueryShape& q, double* d_dist, int* d_idx);
+                      int nrows, const QueryShape& q, double* d_dist, int* d_idx,
+                      Carve* cv = nullptr);
 
 double clock_ms(const mdsctk_knn_ctx* c);
 

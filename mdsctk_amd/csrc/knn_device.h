@@ -15,12 +15,11 @@ constexpr int kTile = 32;  // frames per MFMA tile edge (v_mfma_*_32x32xK)
 constexpr int kRefTilesPerWG = 2;
 // Frame sets are padded to a multiple of this many tiles.
 constexpr int kTileAlign = 4;
-// List capacity of a self query through the score matrix (one list per
-// row): the merge re-scores at most this many candidates per row.
-constexpr int kSymListCap = 2048;
-// Largest k+1 a self query takes through the score matrix when the
-// candidate lists cannot serve it (their limit is 896).
-constexpr int kSymMaxK1 = 1536;
+// Symmetric self query: capacity of each frame's mirrored list (the scores
+// it receives as the reference side of a pair), and the stride of the
+// landmark pass (every kLandmarkStride-th reference tile pair).
+constexpr int kMirrorCap = 8192;
+constexpr int kLandmarkStride = 128;
 
 enum Prec : int { kFP32 = 0, kBF16 = 1, kFP16 = 2, kExact = 3 };
 
@@ -107,6 +106,16 @@ struct Lists {
   unsigned* thr = nullptr;    // [n_fit] shared threshold (float bits)
 };
 
+// The symmetric self query's second list per frame: the scores a frame
+// receives as the reference side of a pair contracted only the other way
+// round ([n][cap] entries, counters, smallest key dropped past cap).
+struct Mirror {
+  uint2* ent = nullptr;
+  int* cnt = nullptr;
+  float* ovf = nullptr;
+  int cap = 0;
+};
+
 // Launchers (all asynchronous on `st`).
 // The prep pass: the FP64 centroids, G and G~ (one wave per frame), then,
 // if f.op is set, the operand (one workgroup per tile and 4 K-groups).
@@ -116,40 +125,56 @@ struct Lists {
 // set's query-side operand).
 hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side,
                        const DevFrames* second, int side2, hipStream_t st);
-hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st);
+// reset_thr false keeps the shared thresholds (a landmark pass set them).
+hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st,
+                             bool reset_thr = true);
 // eps2[q] = 2 kappa (Gt_fit[q] + *ref.gmax); *n_cand (if given) is set to 0.
 hipError_t launch_eps(const DevFrames& fit, const DevFrames& ref, double kappa, float* eps2,
                       int* n_cand, hipStream_t st);
 // Approximate contraction into candidate lists.  When dbg_scores != nullptr
 // it instead writes the approximate score of every pair to
-// dbg_scores[fit*n_ref+ref] (small sizes).  When sym != nullptr (a self
-// query: fit is the reference set's query view) it contracts the upper block
-// triangle only and writes every pair's FP32 score to sym[fit*sym_ld+ref]
-// (sym_ld >= n, a multiple of 4; 16-byte aligned).
+// dbg_scores[fit*n_ref+ref] (small sizes).  With `mirror` (a self query: fit
+// is the reference set's query view) it contracts the upper block triangle
+// only: each pair joins the fit frame's lists and, contracted one way round
+// only, the reference frame's mirrored list when it lies below that
+// frame's shared threshold L.thr.  ph_run < ref_phases walks phases
+// 0..ph_run-1 only (every ref_phases-th reference tile pair: the landmark
+// pass); 0 = all.
 hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lists& L,
                            const float* eps2, int k1, int ref_phases, bool nofit,
                            double* dbg_scores, int grid_wgs, hipStream_t st,
-                           float* sym = nullptr, int64_t sym_ld = 0);
-// The window of a self query through the score matrix (fit: the reference
-// set's query view): eps2[r] = 2 kappa max(G~q[r] + max G~ref,
-// G~ref[r] + max G~q); *gmax_q receives max G~q.
+                           const Mirror* mirror = nullptr, int ph_run = 0);
+// The window of a symmetric self query (fit: the reference set's query
+// view): eps2[r] = 2 kappa max(G~q[r] + max G~ref, G~ref[r] + max G~q);
+// *gmax_q receives max G~q.
 hipError_t launch_eps_sym(const DevFrames& fit, const DevFrames& ref, double kappa,
                           double* gmax_q, float* eps2, int* n_cand, hipStream_t st);
-// Merge order for a self query through the score matrix (n >= 64): rows
-// grouped by their nearest of 64 evenly spaced frames.  label and order
-// hold n ints, counts 64.
-hipError_t launch_sym_order(const float* sym, int64_t sym_ld, int64_t n, int* label,
-                            int* counts, int* order, hipStream_t st);
-// Candidate lists (L.nl == 1) from the n x n score matrix of a self query:
-// per row, every key up to the window of its k1-th smallest key.
-hipError_t launch_sym_select(const float* sym, int64_t sym_ld, int64_t n, const float* eps2,
-                             int k1, const Lists& L, hipStream_t st);
+// After the landmark pass (lists L of n rows, reference phases `phases`):
+// thr[q] = the certified bound of row q's k1-th smallest key in the full
+// query, label[q] = its nearest landmark (dense, < 64 ceil(pairs / phases)).
+hipError_t launch_landmark_cap(const Lists& L, int64_t n, int k1, const float* eps2,
+                               int phases, unsigned* thr, int* label, hipStream_t st);
+// order[0..n) = rows grouped by label (counting sort; counts holds nlab ints).
+hipError_t launch_row_order(const int* label, int64_t n, int nlab, int* counts, int* order,
+                            hipStream_t st);
+hipError_t launch_mirror_init(const Mirror& m, int64_t n, hipStream_t st);
+// out[0] = total mirrored appends (including those dropped past cap), out[1] = max per frame.
+hipError_t launch_mirror_sum(const Mirror& m, int64_t n, unsigned long long* out,
+                             hipStream_t st);
+// dst[i] = frame rows[i] of src (natoms x 3 floats each).
+hipError_t launch_gather_frames(const float* src, const int* rows, int nrows, int natoms,
+                                float* dst, hipStream_t st);
+// Output row i of (sd, si) to row rows[i] of (dd, di), k_out entries each.
+hipError_t launch_scatter_rows(const double* sd, const int* si, const int* rows, int nrows,
+                               int k_out, double* dd, int* di, hipStream_t st);
 // order: the query rows in the order the merge workgroups take them
-// (nullptr: 0..n_fit-1).
+// (nullptr: 0..n_fit-1).  mirror: a symmetric self query's second list per
+// row (nullptr: none).
 hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
                         const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
                         bool nofit, double* out_dist, int* out_idx, int* fallback,
-                        int* n_cand, const int* order, hipStream_t st);
+                        int* n_cand, const int* order, hipStream_t st,
+                        const Mirror* mirror = nullptr);
 // Exact FP64 scores s = G_a + G_b - 2 lambda for fit rows [row0,row0+nrows)
 // against all refs.  Writes s into srow[(row-row0)*n_ref + ref] unless
 // out_full is given, in which case it writes Angstrom distances into

@@ -429,8 +429,17 @@ struct ContractParams {
   int nl, cap;
   int nofit;
   double* dbg;
-  float* sym;      // kOutSym: the score matrix, row stride sym_ld
-  int64_t sym_ld;
+  // Reference phases actually walked: items are (fit block, phase) for
+  // phases 0..ph_run-1 (ph_run < phases: every phases-th reference tile
+  // pair only, the landmark pass of a symmetric self query).
+  int ph_run;
+  // kOutSym: one list per reference frame for the scores it receives as
+  // the reference side of a pair (mcap entries each, appended through the
+  // atomic counter mcnt; the smallest key dropped past mcap goes to movf).
+  uint2* ment;
+  int* mcnt;
+  float* movf;
+  int mcap;
 };
 
 // What the contraction's epilogue produces.
@@ -438,9 +447,11 @@ constexpr int kOutLists = 0;  // per-query candidate lists (the fused pipeline)
 constexpr int kOutDebug = 1;  // every score in FP64 (dbg, n_fit x n_ref)
 // A self query (the query set is the reference set, tile for tile): only
 // the upper block triangle is contracted -- reference pairs that start at
-// or after the workgroup's fit block -- and each score is stored in FP32 at
-// sym[f][r] and, off the diagonal blocks, at sym[r][f] as well, so every
-// entry of the n x n matrix is written exactly once.
+// or after the workgroup's fit block.  Each score joins the fit frame's own
+// candidate list as in kOutLists and, off the diagonal blocks (where the
+// pair is contracted both ways round), the reference frame's list as well
+// when it lies below that frame's shared threshold.  No score matrix is
+// written.
 constexpr int kOutSym = 2;
 
 // One K-group of MFMA work for a wave: acc[c*3+d] += A_c * B_d^T over the
@@ -670,6 +681,31 @@ __device__ __forceinline__ void stage_barrier() {
   asm volatile("" ::: "memory");
 }
 
+// Append this lane's mirrored score (pass: the key is below the reference
+// frame's threshold) to the reference frame's list.  For a fixed accumulator
+// element, the lanes of a half-wave hold the same reference frame (row) and
+// 32 different fit frames, so each half-wave reserves its slots with one
+// atomic add on that row's counter and its passing lanes write in lane
+// order.  b: the wave's ballot of pass.
+__device__ __forceinline__ void mirror_append(uint2* __restrict__ ment, int* __restrict__ mcnt,
+                                              float* __restrict__ movf, int mcap, int64_t row,
+                                              bool pass, uint64_t b, float key,
+                                              unsigned other, int lane) {
+  const int h = lane >> 5;
+  const unsigned hb = h ? (unsigned)(b >> 32) : (unsigned)b;
+  const int lead = hb ? (int)__builtin_ctz(hb) + 32 * h : lane;
+  int base = 0;
+  if (hb && lane == lead) base = atomicAdd(&mcnt[row], __builtin_popcount(hb));
+  base = __shfl(base, lead);
+  if (pass) {
+    const int pos = base + __builtin_popcount(hb & ((1u << (lane & 31)) - 1u));
+    if (pos < mcap)
+      ment[row * (int64_t)mcap + pos] = make_uint2(__float_as_uint(key), other);
+    else
+      atomicMin(reinterpret_cast<unsigned*>(&movf[row]), __float_as_uint(key));
+  }
+}
+
 // FT = fit tiles per workgroup (2 or 4); the workgroup has 2*FT waves, one
 // per (reference tile, fit tile) pair.  MODE: kOutLists, kOutDebug or
 // kOutSym.
@@ -686,6 +722,9 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
   __shared__ uint4 sbuf1[STAGE_VEC];
   __shared__ uint4 sbuf2[STAGE_VEC];
   __shared__ double sGr[2][2 * kTile];  // G~ of the current / next reference tile pair
+  // kOutSym: shared thresholds of the current / next reference tile pair's
+  // frames (the filter of their mirrored scores)
+  __shared__ float sThr[2][2 * kTile];
 
   const int tid = threadIdx.x;
   // wave-uniform, so that everything derived from it (tile roles, copy
@@ -719,14 +758,19 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
   // phases) land on the same XCD and share its L2.
   const int nwg = gridDim.x;
   const int logical = (int)(blockIdx.x & 7) * (nwg >> 3) + (int)(blockIdx.x >> 3);
-  const int n_items = fit_blocks * P.phases;
+  const int n_items = fit_blocks * P.ph_run;
+
+  // The shared threshold of reference frame r (kOutSym), or +inf past the set.
+  auto ref_thr = [&](int64_t r) -> float {
+    return r < P.n_ref ? __uint_as_float(P.thr[r]) : INFINITY;
+  };
 
   const int rounds = (n_items + nwg - 1) / nwg;
   for (int round = 0; round < rounds; ++round) {
     const int item = logical + round * nwg;
     if (item >= n_items) break;
-    const int fb = item / P.phases;
-    const int ph = item - fb * P.phases;
+    const int fb = item / P.ph_run;
+    const int ph = item - fb * P.ph_run;
     const int n_tiles = (P.ref_pairs - ph + P.phases - 1) / P.phases;  // ref pairs ph, ph+R, ...
     // First tile of the item: 0, or for kOutSym the first reference pair at
     // or after the fit block (pair p holds frames 64p.., the block 32*FT*fb..).
@@ -768,20 +812,28 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
                                          (lds_void_t*)(dst + j * 64), 16, 0, 0);
     };
 
-    __syncthreads();  // previous item's readers of sbuf*/sGr are done
-    if (tid < 2 * kTile)
-      sGr[t0 & 1][tid] = P.rGt[(int64_t)(ph + t0 * P.phases) * 2 * kTile + tid];
+    __syncthreads();  // previous item's readers of sbuf*/sGr/sThr are done
+    if (tid < 2 * kTile) {
+      const int64_t f0 = (int64_t)(ph + t0 * P.phases) * 2 * kTile + tid;
+      sGr[t0 & 1][tid] = P.rGt[f0];
+      if constexpr (MODE == kOutSym) sThr[t0 & 1][tid] = ref_thr(f0);
+    }
     dma_stage(t0, 0, d0);
     dma_stage(t0, 1, d1);
     stage_barrier<BPG>();
 
     for (int t = t0; t < n_tiles; ++t) {
       const int p = ph + t * P.phases;
-      // Loads consumed only at the end of the tile: the next tile's G~ and
-      // the shared per-query threshold.
+      // Loads consumed only at the end of the tile: the next tile's G~ (and
+      // for kOutSym its frames' thresholds) and the shared per-query
+      // threshold.
       double g_next = 0.0;
-      if (tid < 2 * kTile && t + 1 < n_tiles)
-        g_next = P.rGt[(int64_t)(p + P.phases) * 2 * kTile + tid];
+      float thr_next = INFINITY;
+      if (tid < 2 * kTile && t + 1 < n_tiles) {
+        const int64_t f1 = (int64_t)(p + P.phases) * 2 * kTile + tid;
+        g_next = P.rGt[f1];
+        if constexpr (MODE == kOutSym) thr_next = ref_thr(f1);
+      }
       unsigned thr_shared = kInfBits;
       if (fvalid && P.thr) thr_shared = __builtin_nontemporal_load(&P.thr[fit_idx]);
 
@@ -805,73 +857,49 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
         dma_stage(t, g + 4, d1);
         stage_barrier<BPG>();
       }
-      // Publish the next tile's G~ (read by its epilogue, after at least one
-      // more barrier).
-      if (tid < 2 * kTile && t + 1 < n_tiles) sGr[(t + 1) & 1][tid] = g_next;
+      // Publish the next tile's G~ and thresholds (read by its epilogue,
+      // after at least one more barrier).
+      if (tid < 2 * kTile && t + 1 < n_tiles) {
+        sGr[(t + 1) & 1][tid] = g_next;
+        if constexpr (MODE == kOutSym) sThr[(t + 1) & 1][tid] = thr_next;
+      }
       if (fvalid && P.thr) thr = fminf(thr, __uint_as_float(thr_shared));
 
       // Epilogue: lane holds fit frame (lane%32) against 16 ref frames.
       const double* const gr_tile = sGr[t & 1];
       const int64_t ref_base = (int64_t)(kRefTilesPerWG * p + ri) * kTile;
-      if constexpr (MODE == kOutSym) {
-        // Scores r = 4j..4j+3 of a lane are ref frames 8j+4h .. 8j+4h+3
-        // (h = lane/32): one 16-byte store per j in the query's row; the
-        // mirrored stores of a half-wave fill 32 consecutive entries of a
-        // reference frame's row.
-        const bool mirror = p >= (fb + 1) * FT / kRefTilesPerWG;
-        float key[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int rl = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-          float m[9];
-#pragma unroll
-          for (int q = 0; q < 9; ++q) m[q] = acc[q][r];
-          // sign bit cleared: the select pass orders keys by their bits
-          key[r] = __uint_as_float(
-              __float_as_uint((float)fmax(
-                  qcp_score_approx(m, gr_tile[ri * kTile + rl], gf, INFINITY, P.nofit), 0.0)) &
-              0x7fffffffu);
-        }
-        if (fvalid) {
-          float* const row = P.sym + fit_idx * P.sym_ld;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int64_t c0 = ref_base + 8 * j + 4 * (lane >> 5);
-            if (c0 + 3 < P.n_ref) {
-              *reinterpret_cast<float4*>(row + c0) =
-                  make_float4(key[4 * j], key[4 * j + 1], key[4 * j + 2], key[4 * j + 3]);
-            } else {
-#pragma unroll
-              for (int e = 0; e < 4; ++e)
-                if (c0 + e < P.n_ref) row[c0 + e] = key[4 * j + e];
-            }
-          }
-          if (mirror) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-              const int64_t ref_idx = ref_base + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-              if (ref_idx < P.n_ref) P.sym[ref_idx * P.sym_ld + fit_idx] = key[r];
-            }
-          }
-        }
-        continue;
-      }
+      // kOutSym: a reference pair wholly past the fit block is contracted
+      // only this way round, so its scores also go to the reference frames'
+      // lists.
+      bool mirror = false;
+      if constexpr (MODE == kOutSym) mirror = p >= (fb + 1) * FT / kRefTilesPerWG;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int rl = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const int64_t ref_idx = ref_base + rl;
-        if (!fvalid || ref_idx >= P.n_ref) continue;
+        const bool valid = fvalid && ref_idx < P.n_ref;
         float m[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) m[q] = acc[q][r];
         const double gr = gr_tile[ri * kTile + rl];
         if constexpr (MODE == kOutDebug) {
-          P.dbg[fit_idx * P.n_ref + ref_idx] = qcp_score_approx(m, gr, gf, INFINITY, P.nofit);
+          if (valid)
+            P.dbg[fit_idx * P.n_ref + ref_idx] = qcp_score_approx(m, gr, gf, INFINITY, P.nofit);
           continue;
         }
-        const double s = qcp_score_approx(m, gr, gf, (double)thr, P.nofit);
-        if (s <= (double)thr) {
-          const float key = (float)fmax(s, 0.0);
+        // Newton stops once the pair is out of every list it could join.
+        float tr = -1.0f;  // the reference frame's threshold (mirrored scores only)
+        if constexpr (MODE == kOutSym)
+          if (mirror) tr = sThr[t & 1][ri * kTile + rl];
+        bool own = false, mir = false;
+        float key = 0.f;
+        if (valid) {
+          const double s = qcp_score_approx(m, gr, gf, fmax((double)thr, (double)tr), P.nofit);
+          key = (float)fmax(s, 0.0);
+          own = s <= (double)thr;
+          mir = s <= (double)tr;
+        }
+        if (own) {
           // The compaction keeps cnt <= cap - 16 between tiles, so the guard
           // never drops; if it did, ovf would send the row to the exact path.
           if (cnt < P.cap) {
@@ -881,6 +909,12 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
             ovf = fminf(ovf, key);
           }
         }
+        if constexpr (MODE == kOutSym) {
+          const uint64_t b = __ballot(mir);
+          if (b)
+            mirror_append(P.ment, P.mcnt, P.movf, P.mcap, ref_idx, mir, b, key,
+                          (unsigned)fit_idx, lane);
+        }
       }
       if constexpr (MODE == kOutDebug) continue;
       const float thr_before = thr;
@@ -888,7 +922,7 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
       thr = fminf(thr, __shfl_xor(thr, 32));
       if (fvalid && P.thr && thr < thr_before) atomicMin(&P.thr[fit_idx], __float_as_uint(thr));
     }
-    if (fvalid && MODE == kOutLists) {
+    if (fvalid && MODE != kOutDebug) {
       P.cnt[hdr] = cnt;
       P.ovf[hdr] = ovf;
     }
@@ -1014,6 +1048,12 @@ struct MergeParams {
   const int* cnt;
   const float* ovf;
   int nl, cap;
+  // a symmetric self query's second list per row (the scores the row
+  // received as the reference side of a pair); null otherwise
+  const uint2* ment;
+  const int* mcnt;
+  const float* movf;
+  int mcap;
   const float* eps2;
   int k1, k_out, rank_off;
   int nofit;
@@ -1066,8 +1106,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
   const int* cnt = P.cnt + q * P.nl;
   const uint2* ent = P.ent + q * P.nl * (int64_t)P.cap;
   const float* ovf = P.ovf + q * P.nl;
+  const int mc = P.ment ? min(P.mcnt[q], P.mcap) : 0;
+  const uint2* ment = P.ment ? P.ment + q * (int64_t)P.mcap : nullptr;
+  // f(entry) for every entry of the row's lists (waves over the lists,
+  // lanes over entries; the whole workgroup over the mirrored list)
+  auto visit = [&](auto&& f) {
+    for (int l = wave; l < P.nl; l += 4) {
+      const int c = cnt[l];
+      for (int i = lane; i < c; i += 64) f(ent[(int64_t)l * P.cap + i]);
+    }
+    for (int i = tid; i < mc; i += 256) f(ment[i]);
+  };
 
-  if (tid == 0) { s_total = 0; s_nc = 0; s_flag = 0; s_bad = 0; }
+  if (tid == 0) { s_total = mc; s_nc = 0; s_flag = 0; s_bad = 0; }
   __syncthreads();
   {
     int part = 0;
@@ -1086,13 +1137,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
     hist[tid] = 0;
     __syncthreads();
     const unsigned mhi = (shift == 24) ? 0u : (0xFFFFFFFFu << (shift + 8));
-    for (int l = wave; l < P.nl; l += 4) {
-      const int c = cnt[l];
-      for (int i = lane; i < c; i += 64) {
-        const unsigned k = ent[(int64_t)l * P.cap + i].x;
-        if ((k & mhi) == (prefix & mhi)) atomicAdd(&hist[(k >> shift) & 255u], 1u);
-      }
-    }
+    visit([&](const uint2 e) {
+      const unsigned k = e.x;
+      if ((k & mhi) == (prefix & mhi)) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+    });
     __syncthreads();
     if (tid == 0) {
       unsigned cum = 0;
@@ -1112,19 +1160,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
   const float cut = tau * (1.0f + 2.4e-7f) + P.eps2[q];
   for (int l = tid; l < P.nl; l += 256)
     if (ovf[l] <= cut) s_flag = 1;
-  for (int l = wave; l < P.nl; l += 4) {
-    const int c = cnt[l];
-    for (int i = lane; i < c; i += 64) {
-      const uint2 e = ent[(int64_t)l * P.cap + i];
-      if (__uint_as_float(e.x) <= cut) {
-        const int pos = atomicAdd(&s_nc, 1);
-        if (pos < CAND) {
-          cand_idx[pos] = (int)e.y;
-          cand_key[pos] = __uint_as_float(e.x);
-        }
+  if (tid == 0 && P.ment && P.movf[q] <= cut) s_flag = 1;
+  visit([&](const uint2 e) {
+    if (__uint_as_float(e.x) <= cut) {
+      const int pos = atomicAdd(&s_nc, 1);
+      if (pos < CAND) {
+        cand_idx[pos] = (int)e.y;
+        cand_key[pos] = __uint_as_float(e.x);
       }
     }
-  }
+  });
   __syncthreads();
   const int nc = s_nc;
   if (s_flag || nc > CAND) {
@@ -1177,312 +1222,204 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
 }
 
 // ---------------------------------------------------------------------------
-// Self query through the score matrix: each query row's candidates, in the
-// candidate-list form the merge reads (one list per row).
+// Symmetric self query: thresholds from the landmark pass, merge order.
 
-constexpr int kSymBins = 4096;
-constexpr int kSymStage = 4096;       // keys staged in LDS by the second pass
-constexpr int kSymStageSmall = 2048;  // the instance for k1 <= kMergeSmallK1
-
-// Calls f(index, key bits) for every key of a row (256 threads): 16-byte
-// loads, four in flight per thread, over the aligned part, then the tail.
-template <typename F>
-__device__ __forceinline__ void sym_row_visit(const float* __restrict__ row, int64_t n, F&& f) {
-  const int64_t n4 = n >> 2;
-  const float4* __restrict__ r4 = reinterpret_cast<const float4*>(row);
-  int64_t i = threadIdx.x;
-  for (; i + 3 * 256 < n4; i += 4 * 256) {
-    float4 v[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = r4[i + j * 256];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t e = 4 * (i + j * 256);
-      f(e, __float_as_uint(v[j].x));
-      f(e + 1, __float_as_uint(v[j].y));
-      f(e + 2, __float_as_uint(v[j].z));
-      f(e + 3, __float_as_uint(v[j].w));
-    }
-  }
-  for (; i < n4; i += 256) {
-    const float4 v = r4[i];
-    f(4 * i, __float_as_uint(v.x));
-    f(4 * i + 1, __float_as_uint(v.y));
-    f(4 * i + 2, __float_as_uint(v.z));
-    f(4 * i + 3, __float_as_uint(v.w));
-  }
-  for (int64_t e = 4 * n4 + threadIdx.x; e < n; e += 256) f(e, __float_as_uint(row[e]));
-}
-
-// The bin of a 4096-bin histogram that holds rank *kk (0-based); *kk
-// becomes the rank within that bin.  256 threads, 16 bins each.
-__device__ int sym_find_bin(const unsigned* hist, unsigned* scan, unsigned& kk) {
-  __shared__ int s_bin;
-  __shared__ unsigned s_rank;
-  const int tid = threadIdx.x;
-  unsigned local = 0;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) local += hist[tid * 16 + j];
-  scan[tid] = local;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {  // inclusive Hillis-Steele scan
-    const unsigned v = tid >= off ? scan[tid - off] : 0u;
-    __syncthreads();
-    scan[tid] += v;
-    __syncthreads();
-  }
-  const unsigned incl = scan[tid], excl = incl - local;
-  if (excl <= kk && kk < incl) {
-    unsigned cum = excl;
-    int b = tid * 16;
-    while (cum + hist[b] <= kk) cum += hist[b++];
-    s_bin = b;
-    s_rank = kk - cum;
-  }
-  __syncthreads();
-  kk = s_rank;
-  return s_bin;
-}
-
-// Upper edge (the first float above) of fine bin b2 inside coarse bin b1.
-__device__ __forceinline__ unsigned sym_bin_top(unsigned b1, unsigned b2) {
-  return (((b1 << 12) | b2) + 1u) << 7;
-}
-
-// The window bound of a key bound: top (1 + 2.4e-7) + eps2, the merge's
-// window formula (monotone in its argument).  Keys are finite, so a top
-// at or beyond the infinity bits only arises from a bound that covers all.
-__device__ __forceinline__ unsigned sym_window_bits(unsigned top, float e2) {
-  const float w = top >= kInfBits ? INFINITY : __uint_as_float(top) * (1.0f + 2.4e-7f) + e2;
-  return __float_as_uint(w);
-}
-
-constexpr int64_t kSymSampleMin = 32768;  // rows at least this long sample first
-
-// One 256-thread workgroup per query row q of the n x n FP32 score matrix:
-// the row's candidate list for the merge.  Keys are non-negative floats, so
-// their bits order like the values; tau is the row's k1-th smallest key, and
-// the merge's window is tau (1 + 2.4e-7) + eps2, monotone in tau.  A bin
-// below is 2^7 ulps wide (key bits 30..19 coarse, 18..7 fine).
-//
-// Sampled form (rows of kSymSampleMin keys or more): a sample of 1/16 of
-// the row (16 evenly spaced runs) gives T, the top of the fine bin of its
-// rank-r key, r about twice k1/16.  One pass over the row stages every key
-// up to the window of T in LDS and counts the keys below T.  If at least k1
-// are below T, then tau < T, every key up to the window of tau is staged,
-// and tau's bin and window are found among the staged keys alone.
-//
-// Full form (short rows, or the sampled form's bound failed or staged too
-// many keys): a histogram of the whole row finds tau's coarse bin b1; a
-// second pass lists the keys below b1 (all <= tau) and stages those from b1
-// up to the window of b1's top; the fine bin comes from the staged keys
-// (or, past STAGE_N of them, from a third pass over the row, which then
-// also gathers).
-//
-// Every key up to the window of tau's bin top joins the list, so the list
-// holds the merge's whole window.  Past `cap` entries the smallest dropped
-// key goes to ovf, which makes the merge take the exact path if it falls
-// inside the window.
-template <int STAGE_N>
-__global__ __launch_bounds__(256) void sym_select_kernel(const float* __restrict__ S, int64_t ld,
-                                                         int64_t n, const float* __restrict__ eps2,
-                                                         int k1, uint2* __restrict__ ent,
-                                                         int* __restrict__ cnt,
-                                                         float* __restrict__ ovf, int cap) {
-  __shared__ unsigned hist[kSymBins];
-  __shared__ unsigned scan[256];
-  __shared__ uint2 stage[STAGE_N];
-  __shared__ int s_n, s_ns, s_nt;
-  __shared__ unsigned s_ovf;
+// One 256-thread workgroup per row q of the landmark pass (the contraction
+// of every frame against every phases-th reference tile pair, kOutLists
+// into nl lists per row).  Its k1-th smallest key tau_L bounds the row's
+// k1-th smallest key of the full query: k1 landmark pairs have keys <= tau_L,
+// so their exact scores are <= tau_L + eps and their keys in the full query
+// (computed either way round) <= tau_L + 2 eps = tau_L + e2.  thr[q] becomes
+// the window of that bound, (tau_L + e2) (1 + 2.4e-7) + e2 rounded up with
+// slack; with fewer than k1 landmark keys it is +inf.  label[q]: the nearest
+// landmark (smallest key, then smallest index) as a dense number
+// 64 j + o for the landmark frame 64 (j R) + o (R = phases).
+__global__ __launch_bounds__(256) void landmark_cap_kernel(const uint2* __restrict__ ent,
+                                                           const int* __restrict__ cnt, int nl,
+                                                           int cap, int k1,
+                                                           const float* __restrict__ eps2,
+                                                           int phases, unsigned* __restrict__ thr,
+                                                           int* __restrict__ label) {
   const int64_t q = blockIdx.x;
-  const float* const row = S + q * ld;
-  const int tid = threadIdx.x;
-  const float e2 = eps2[q];
-  uint2* const out = ent + q * (int64_t)cap;
-  auto put = [&](unsigned u, unsigned i) {
-    const int pos = atomicAdd(&s_n, 1);
-    if (pos < cap)
-      out[pos] = make_uint2(u, i);
-    else
-      atomicMin(&s_ovf, u);
-  };
-  auto clear_hist = [&]() {
-    for (int i = tid; i < kSymBins; i += 256) hist[i] = 0u;
-  };
-  // tau's fine bin among the staged keys [0, ns) (all of the row's keys up
-  // to the staged range's top must be there); then the list from them
-  auto select_staged = [&](int ns, unsigned kk) {
-    clear_hist();
-    __syncthreads();
-    for (int p = tid; p < ns; p += 256) atomicAdd(&hist[(stage[p].x >> 19) & (kSymBins - 1)], 1u);
-    __syncthreads();
-    const unsigned b1 = (unsigned)sym_find_bin(hist, scan, kk);
-    clear_hist();
-    __syncthreads();
-    for (int p = tid; p < ns; p += 256) {
-      const unsigned u = stage[p].x;
-      if ((u >> 19) == b1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
-    }
-    __syncthreads();
-    const unsigned b2 = (unsigned)sym_find_bin(hist, scan, kk);
-    const unsigned cut_bits = sym_window_bits(sym_bin_top(b1, b2), e2);
-    for (int p = tid; p < ns; p += 256) {
-      const uint2 e = stage[p];
-      if (e.x <= cut_bits) put(e.x, e.y);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  __shared__ unsigned hist[256];
+  __shared__ unsigned s_prefix;
+  __shared__ int s_kk, s_total;
+  __shared__ unsigned long long s_best;
+  const int* c = cnt + q * nl;
+  const uint2* e = ent + q * (int64_t)nl * cap;
+  auto visit = [&](auto&& f) {
+    for (int l = wave; l < nl; l += 4) {
+      const int cl = c[l];
+      for (int i = lane; i < cl; i += 64) f(e[(int64_t)l * cap + i]);
     }
   };
-
   if (tid == 0) {
-    s_n = 0;
-    s_ns = 0;
-    s_nt = 0;
-    s_ovf = kInfBits;
+    s_total = 0;
+    s_best = ~0ull;
   }
-  bool done = false;
-  if (n >= kSymSampleMin) {
-    const int c = (int)(n / 256), stride = (int)(n / 16), m = 16 * c;
-    auto sample_key = [&](int t) {
-      const int j = t / c;
-      return __float_as_uint(row[(int64_t)j * stride + (t - j * c)]);
-    };
-    unsigned kr = (unsigned)min((int64_t)m - 1, 2 * (((int64_t)k1 * m + n - 1) / n) + 8);
-    clear_hist();
-    __syncthreads();
-    for (int t = tid; t < m; t += 256) atomicAdd(&hist[(sample_key(t) >> 19) & (kSymBins - 1)], 1u);
-    __syncthreads();
-    const unsigned sb1 = (unsigned)sym_find_bin(hist, scan, kr);
-    clear_hist();
-    __syncthreads();
-    for (int t = tid; t < m; t += 256) {
-      const unsigned u = sample_key(t);
-      if ((u >> 19) == sb1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
-    }
-    __syncthreads();
-    const unsigned sb2 = (unsigned)sym_find_bin(hist, scan, kr);
-    const unsigned top = sym_bin_top(sb1, sb2);
-    const unsigned w_bits = sym_window_bits(top, e2);
-    sym_row_visit(row, n, [&](int64_t i, unsigned u) {
-      if (u <= w_bits) {
-        if (u < top) atomicAdd(&s_nt, 1);
-        const int p = atomicAdd(&s_ns, 1);
-        if (p < STAGE_N) stage[p] = make_uint2(u, (unsigned)i);
-      }
+  __syncthreads();
+  {
+    int part = 0;
+    unsigned long long best = ~0ull;
+    for (int l = tid; l < nl; l += 256) part += c[l];
+    atomicAdd(&s_total, part);
+    visit([&](const uint2 x) {
+      const unsigned long long kx = ((unsigned long long)x.x << 32) | x.y;
+      best = kx < best ? kx : best;
     });
-    __syncthreads();
-    const int ns = s_ns;
-    if (s_nt >= k1 && ns <= STAGE_N) {
-      select_staged(ns, (unsigned)(k1 - 1));
-      done = true;
-    }
-    __syncthreads();
-    if (tid == 0) s_ns = 0;
-  }
-
-  if (!done) {
-    unsigned kk = (unsigned)(k1 - 1);
-    clear_hist();
-    __syncthreads();
-    sym_row_visit(row, n,
-                  [&](int64_t, unsigned u) { atomicAdd(&hist[(u >> 19) & (kSymBins - 1)], 1u); });
-    __syncthreads();
-    const unsigned b1 = (unsigned)sym_find_bin(hist, scan, kk);
-    const unsigned lo1 = b1 << 19;
-    const unsigned c1_bits = sym_window_bits((b1 + 1u) << 19, e2);
-    sym_row_visit(row, n, [&](int64_t i, unsigned u) {
-      if (u < lo1) {
-        put(u, (unsigned)i);
-      } else if (u <= c1_bits) {
-        const int p = atomicAdd(&s_ns, 1);
-        if (p < STAGE_N) stage[p] = make_uint2(u, (unsigned)i);
-      }
-    });
-    clear_hist();
-    __syncthreads();
-    const int ns = s_ns;
-    if (ns <= STAGE_N) {
-      for (int p = tid; p < ns; p += 256) {
-        const unsigned u = stage[p].x;
-        if ((u >> 19) == b1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
-      }
-    } else {
-      sym_row_visit(row, n, [&](int64_t, unsigned u) {
-        if ((u >> 19) == b1) atomicAdd(&hist[(u >> 7) & (kSymBins - 1)], 1u);
-      });
-    }
-    __syncthreads();
-    const unsigned b2 = (unsigned)sym_find_bin(hist, scan, kk);
-    const unsigned cut_bits = sym_window_bits(sym_bin_top(b1, b2), e2);
-    if (ns <= STAGE_N) {
-      for (int p = tid; p < ns; p += 256) {
-        const uint2 e = stage[p];
-        if (e.x <= cut_bits) put(e.x, e.y);
-      }
-    } else {
-      sym_row_visit(row, n, [&](int64_t i, unsigned u) {
-        if (u >= lo1 && u <= cut_bits) put(u, (unsigned)i);
-      });
-    }
+    atomicMin(&s_best, best);
   }
   __syncthreads();
   if (tid == 0) {
-    cnt[q] = min(s_n, cap);
-    ovf[q] = __uint_as_float(s_ovf);
-  }
-}
-
-// Processing order for the merge of a symmetric self query: rows grouped by
-// the nearest of kSymLandmarks evenly spaced reference frames (read from the
-// score matrix), so that rows merged at the same time share their
-// candidates' frames in L2 and MALL.  The order only changes which
-// workgroup takes which row, never a result.
-constexpr int kSymLandmarks = 64;
-
-__global__ void sym_order_zero_kernel(int* __restrict__ counts) {
-  if (threadIdx.x < kSymLandmarks) counts[threadIdx.x] = 0;
-}
-
-__global__ void sym_label_kernel(const float* __restrict__ S, int64_t ld, int64_t n,
-                                 int* __restrict__ label, int* __restrict__ counts) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const int64_t stride = n / kSymLandmarks;
-  const float* const row = S + q * ld;
-  unsigned best = 0xffffffffu;
-  int arg = 0;
-  for (int j = 0; j < kSymLandmarks; ++j) {
-    const unsigned u = __float_as_uint(row[j * stride]);
-    if (u < best) {
-      best = u;
-      arg = j;
+    int lab = 0;
+    if (s_best != ~0ull) {
+      const unsigned r = (unsigned)(s_best & 0xffffffffu);
+      const unsigned span = 2u * kTile * (unsigned)phases;
+      lab = (int)((r / span) * (2u * kTile) + r % (2u * kTile));
     }
+    label[q] = lab;
   }
-  label[q] = arg;
-  atomicAdd(&counts[arg], 1);
+  if (s_total < k1) {
+    if (tid == 0) thr[q] = kInfBits;
+    return;
+  }
+  unsigned prefix = 0;
+  int kk = k1 - 1;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    const unsigned mhi = (shift == 24) ? 0u : (0xFFFFFFFFu << (shift + 8));
+    visit([&](const uint2 x) {
+      if ((x.x & mhi) == (prefix & mhi)) atomicAdd(&hist[(x.x >> shift) & 255u], 1u);
+    });
+    __syncthreads();
+    if (tid == 0) {
+      unsigned cum = 0;
+      int b = 0;
+      for (; b < 255; ++b) {
+        if (cum + hist[b] > (unsigned)kk) break;
+        cum += hist[b];
+      }
+      s_prefix = prefix | ((unsigned)b << shift);
+      s_kk = kk - (int)cum;
+    }
+    __syncthreads();
+    prefix = s_prefix;
+    kk = s_kk;
+  }
+  if (tid == 0) {
+    const double tau = (double)__uint_as_float(prefix), e2 = (double)eps2[q];
+    const float bound = (float)((tau + e2) * (1.0 + 1e-6) + e2);
+    thr[q] = bound < INFINITY ? __float_as_uint(bound) + 1u : kInfBits;
+  }
 }
 
-// counts -> exclusive offsets, in place (one block of kSymLandmarks threads)
-__global__ void sym_order_scan_kernel(int* __restrict__ counts) {
-  __shared__ int c[kSymLandmarks];
+// Merge order of a symmetric self query: rows grouped by their nearest
+// landmark (a counting sort over nlab labels), so that rows merged at the
+// same time share candidate frames in L2 and MALL.  The order only decides
+// which workgroup takes which row, never a result.
+__global__ void order_count_kernel(const int* __restrict__ label, int64_t n,
+                                   int* __restrict__ counts) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) atomicAdd(&counts[label[q]], 1);
+}
+
+// counts -> exclusive offsets in place; one block of 1024 threads, each
+// over a run of consecutive labels.
+__global__ __launch_bounds__(1024) void order_scan_kernel(int* __restrict__ counts, int nlab) {
+  __shared__ int part[1024];
   const int t = threadIdx.x;
-  c[t] = counts[t];
+  const int per = (nlab + 1023) / 1024;
+  const int lo = min(t * per, nlab), hi = min(lo + per, nlab);
+  int sum = 0;
+  for (int j = lo; j < hi; ++j) sum += counts[j];
+  part[t] = sum;
   __syncthreads();
-  if (t == 0) {
-    int acc = 0;
-    for (int j = 0; j < kSymLandmarks; ++j) {
-      const int v = c[j];
-      c[j] = acc;
-      acc += v;
-    }
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
   }
-  __syncthreads();
-  counts[t] = c[t];
+  int acc = part[t] - sum;
+  for (int j = lo; j < hi; ++j) {
+    const int v = counts[j];
+    counts[j] = acc;
+    acc += v;
+  }
 }
 
-__global__ void sym_order_scatter_kernel(const int* __restrict__ label, int64_t n,
-                                         int* __restrict__ offsets, int* __restrict__ order) {
+__global__ void order_scatter_kernel(const int* __restrict__ label, int64_t n,
+                                     int* __restrict__ offsets, int* __restrict__ order) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  order[atomicAdd(&offsets[label[q]], 1)] = (int)q;
+  if (q < n) order[atomicAdd(&offsets[label[q]], 1)] = (int)q;
+}
+
+__global__ void fill_int_kernel(int* __restrict__ p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// The mirrored lists' counters and overflow keys (set before the symmetric
+// contraction) and, after it, their total fill.
+__global__ void mirror_init_kernel(int* __restrict__ mcnt, float* __restrict__ movf, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    mcnt[i] = 0;
+    movf[i] = INFINITY;
+  }
+}
+
+__global__ void mirror_sum_kernel(const int* __restrict__ mcnt, int64_t n, int mcap,
+                                  unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[256];
+  unsigned long long s = 0, mx = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    const unsigned long long v = (unsigned long long)mcnt[i];
+    s += v;
+    mx = v > mx ? v : mx;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+  __syncthreads();
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k)
+      red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + k] ? red[threadIdx.x]
+                                                                 : red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[1] = red[0];
+  (void)mcap;
+}
+
+// Rows of a frame set gathered into a batch (rows[i] -> dst row i), and
+// output rows scattered back (src row i -> dst row rows[i]).
+__global__ void gather_frames_kernel(const float* __restrict__ src, const int* __restrict__ rows,
+                                     int natoms, float* __restrict__ dst) {
+  const int64_t i = blockIdx.x;
+  const int64_t w = (int64_t)natoms * 3;
+  const float* s = src + (int64_t)rows[i] * w;
+  float* d = dst + i * w;
+  for (int64_t e = threadIdx.x; e < w; e += blockDim.x) d[e] = s[e];
+}
+
+__global__ void scatter_rows_kernel(const double* __restrict__ sd, const int* __restrict__ si,
+                                    const int* __restrict__ rows, int k_out,
+                                    double* __restrict__ dd, int* __restrict__ di) {
+  const int64_t i = blockIdx.x;
+  const int64_t r = rows[i];
+  for (int j = threadIdx.x; j < k_out; j += blockDim.x) {
+    dd[r * k_out + j] = sd[i * k_out + j];
+    di[r * k_out + j] = si[i * k_out + j];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1621,6 +1558,7 @@ __global__ __launch_bounds__(256) void select_rows_kernel(const double* __restri
   }
 }
 
+// thr may be null: the shared thresholds are kept (set by a landmark pass).
 __global__ void init_lists_kernel(int* cnt, float* ovf, unsigned* thr, int64_t n_hdr,
                                   int64_t n_fit) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1628,7 +1566,7 @@ __global__ void init_lists_kernel(int* cnt, float* ovf, unsigned* thr, int64_t n
     cnt[i] = 0;
     ovf[i] = INFINITY;
   }
-  if (i < n_fit) thr[i] = kInfBits;
+  if (thr && i < n_fit) thr[i] = kInfBits;
 }
 
 __global__ void max_kernel(const double* __restrict__ v, int64_t n, double* out) {
@@ -1655,7 +1593,7 @@ __global__ void eps_kernel(const double* __restrict__ fGt, int64_t n_fit,
   if (i < n_fit) eps2[i] = (float)(2.0 * kappa * (fGt[i] + *gmax));
 }
 
-// Self query through the score matrix: row r also receives the scores of
+// Symmetric self query: row r also receives the scores of
 // pairs computed the other way round (r's reference-side operand against
 // f's query-side one), whose error bound is kappa (G~ref[r] + G~q[f]); the
 // window covers both kinds.
@@ -1777,11 +1715,11 @@ hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side,
   return hipGetLastError();
 }
 
-hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st) {
+hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st, bool reset_thr) {
   const int64_t n_hdr = n_fit * L.nl;
   const int64_t n = n_hdr > n_fit ? n_hdr : n_fit;
   hipLaunchKernelGGL(init_lists_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     L.cnt, L.ovf, L.thr, n_hdr, n_fit);
+                     L.cnt, L.ovf, reset_thr ? L.thr : nullptr, n_hdr, n_fit);
   return hipGetLastError();
 }
 
@@ -1813,8 +1751,8 @@ hipError_t launch_eps_sym(const DevFrames& fit, const DevFrames& ref, double kap
 
 hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lists& L,
                            const float* eps2, int k1, int ref_phases, bool nofit,
-                           double* dbg_scores, int grid_wgs, hipStream_t st, float* sym,
-                           int64_t sym_ld) {
+                           double* dbg_scores, int grid_wgs, hipStream_t st, const Mirror* mirror,
+                           int ph_run) {
   ContractParams P;
   P.rop = ref.op;
   P.fop = fit.op;
@@ -1836,9 +1774,14 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   P.cap = L.cap;
   P.nofit = nofit ? 1 : 0;
   P.dbg = dbg_scores;
-  P.sym = sym;
-  P.sym_ld = sym_ld;
-  if (sym && (fit.n != ref.n || fit.n_tiles != ref.n_tiles || sym_ld < ref.n || (sym_ld & 3)))
+  P.ph_run = ph_run > 0 ? ph_run : ref_phases;
+  P.ment = mirror ? mirror->ent : nullptr;
+  P.mcnt = mirror ? mirror->cnt : nullptr;
+  P.movf = mirror ? mirror->ovf : nullptr;
+  P.mcap = mirror ? mirror->cap : 0;
+  if (P.ph_run > ref_phases || ref_phases < 1) return hipErrorInvalidValue;
+  if (mirror && (fit.n != ref.n || fit.n_tiles != ref.n_tiles || !L.thr || !mirror->ent ||
+                 !mirror->cnt || !mirror->ovf || mirror->cap < 1 || P.ph_run != ref_phases))
     return hipErrorInvalidValue;
   dim3 grid(grid_wgs);
   // FP32 runs two 4-wave workgroups per CU (2 x 2 tiles each); the split
@@ -1858,8 +1801,14 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   do {                                              \
     if (dbg_scores)                                 \
       MDSCTK_CONTRACT_TR(PR, 4, FT, kOutDebug);     \
-    else if (sym)                                   \
+    else if (mirror && L.cap == 256)                \
       MDSCTK_CONTRACT_TR(PR, 4, FT, kOutSym);       \
+    else if (mirror && L.cap == 512)                \
+      MDSCTK_CONTRACT_TR(PR, 8, FT, kOutSym);       \
+    else if (mirror && L.cap == 1024)               \
+      MDSCTK_CONTRACT_TR(PR, 16, FT, kOutSym);      \
+    else if (mirror)                                \
+      return hipErrorInvalidValue;                  \
     else if (L.cap == 256)                          \
       MDSCTK_CONTRACT_TR(PR, 4, FT, kOutLists);     \
     else if (L.cap == 512)                          \
@@ -1903,7 +1852,7 @@ static PairSource pair_source(const DevFrames& ref, const DevFrames& fit, const 
 hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
                         const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
                         bool nofit, double* out_dist, int* out_idx, int* fallback, int* n_cand,
-                        const int* order, hipStream_t st) {
+                        const int* order, hipStream_t st, const Mirror* mirror) {
   MergeParams P;
   P.src = pair_source(ref, fit, w);
   P.W = w.W;
@@ -1913,6 +1862,10 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
   P.ovf = L.ovf;
   P.nl = L.nl;
   P.cap = L.cap;
+  P.ment = mirror ? mirror->ent : nullptr;
+  P.mcnt = mirror ? mirror->cnt : nullptr;
+  P.movf = mirror ? mirror->ovf : nullptr;
+  P.mcap = mirror ? mirror->cap : 0;
   P.eps2 = eps2;
   P.k1 = k1;
   P.k_out = k_out;
@@ -1942,31 +1895,55 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
   return hipGetLastError();
 }
 
-hipError_t launch_sym_order(const float* sym, int64_t sym_ld, int64_t n, int* label,
-                            int* counts, int* order, hipStream_t st) {
-  if (n < kSymLandmarks || sym_ld < n) return hipErrorInvalidValue;
+hipError_t launch_landmark_cap(const Lists& L, int64_t n, int k1, const float* eps2,
+                               int phases, unsigned* thr, int* label, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (k1 < 1 || phases < 1 || !thr || !label) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(landmark_cap_kernel, dim3((unsigned)n), dim3(256), 0, st, L.ent, L.cnt, L.nl,
+                     L.cap, k1, eps2, phases, thr, label);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_order(const int* label, int64_t n, int nlab, int* counts, int* order,
+                            hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (nlab < 1) return hipErrorInvalidValue;
   const unsigned blocks = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(sym_order_zero_kernel, dim3(1), dim3(kSymLandmarks), 0, st, counts);
-  hipLaunchKernelGGL(sym_label_kernel, dim3(blocks), dim3(256), 0, st, sym, sym_ld, n, label,
-                     counts);
-  hipLaunchKernelGGL(sym_order_scan_kernel, dim3(1), dim3(kSymLandmarks), 0, st, counts);
-  hipLaunchKernelGGL(sym_order_scatter_kernel, dim3(blocks), dim3(256), 0, st, label, n, counts,
+  hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)((nlab + 255) / 256)), dim3(256), 0, st,
+                     counts, (int64_t)nlab, 0);
+  hipLaunchKernelGGL(order_count_kernel, dim3(blocks), dim3(256), 0, st, label, n, counts);
+  hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(1024), 0, st, counts, nlab);
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(blocks), dim3(256), 0, st, label, n, counts,
                      order);
   return hipGetLastError();
 }
 
-hipError_t launch_sym_select(const float* sym, int64_t sym_ld, int64_t n, const float* eps2,
-                             int k1, const Lists& L, hipStream_t st) {
-  if (L.nl != 1 || k1 < 1 || k1 > n || sym_ld < n || (sym_ld & 3)) return hipErrorInvalidValue;
-  if (n == 0) return hipSuccess;
-  // small k: a 2048-key stage (C3 stages about 300 keys a row), so that
-  // 4 workgroups fit per CU instead of 3
-  if (k1 <= kMergeSmallK1)
-    hipLaunchKernelGGL(sym_select_kernel<kSymStageSmall>, dim3((unsigned)n), dim3(256), 0, st, sym,
-                       sym_ld, n, eps2, k1, L.ent, L.cnt, L.ovf, L.cap);
-  else
-    hipLaunchKernelGGL(sym_select_kernel<kSymStage>, dim3((unsigned)n), dim3(256), 0, st, sym,
-                       sym_ld, n, eps2, k1, L.ent, L.cnt, L.ovf, L.cap);
+hipError_t launch_mirror_init(const Mirror& m, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mirror_init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     m.cnt, m.ovf, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_mirror_sum(const Mirror& m, int64_t n, unsigned long long* out,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(mirror_sum_kernel, dim3(1), dim3(256), 0, st, m.cnt, n, m.cap, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_frames(const float* src, const int* rows, int nrows, int natoms,
+                                float* dst, hipStream_t st) {
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_frames_kernel, dim3((unsigned)nrows), dim3(256), 0, st, src, rows,
+                     natoms, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_rows(const double* sd, const int* si, const int* rows, int nrows,
+                               int k_out, double* dd, int* di, hipStream_t st) {
+  if (nrows <= 0 || k_out <= 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_rows_kernel, dim3((unsigned)nrows), dim3(64), 0, st, sd, si, rows,
+                     k_out, dd, di);
   return hipGetLastError();
 }
 
@@ -2024,5 +2001,6 @@ hipError_t launch_identity_rows(int64_t n_fit, int64_t n_ref, int* out_idx, hipS
                      n_ref, out_idx);
   return hipGetLastError();
 }
+
 
 }  // namespace mdsctk

@@ -11,9 +11,13 @@
 // full unsorted rows [n_fit][n_ref] and identity indices.
 // Options added here (no reference counterpart): --precision
 // {bf16,fp16,fp32,exact} (contraction form, default bf16) and --device N.
-// The pair computation runs on the GPU through libmdsctk_knn; -t sets the
-// XTC decode threads and -b is accepted and reported (the GPU tiles the
-// work itself).
+// The pair computation runs on the GPU through libmdsctk_knn.  The XTC
+// files are decoded on the GPU as they are read (mdsctk_xtc_read_device),
+// so the trajectories never sit in host memory; outputs are produced and
+// written in row blocks of at most 2^27 values (the reference's 2^30 / 8
+// doubles per block, knn_rms.cpp:143, 213-221, 281-291), so host memory stays
+// bounded for --sort false too.  -t and -b are accepted and reported (the
+// GPU decodes and tiles the work itself).
 #include <algorithm>
 #include <cctype>
 #include <chrono>
@@ -24,6 +28,8 @@
 #include <string>
 #include <thread>
 #include <vector>
+
+#include <hip/hip_runtime.h>
 
 #include "cli_options.h"
 #include "mdsctk_io.h"
@@ -139,91 +145,172 @@ int main(int argc, char** argv) {
   }
   std::printf("done.\n");
 
-  auto load = [&](const std::string& path, std::vector<float>& xyz, int64_t* n) -> int {
-    int64_t nf = 0;
+  using clk = std::chrono::steady_clock;
+  auto secs_since = [](clk::time_point t) {
+    return std::chrono::duration<double>(clk::now() - t).count();
+  };
+  const auto t_start = clk::now();
+  // device buffers and pinned host buffers of the run, released at exit
+  struct Held {
+    float* d_ref = nullptr;
+    float* d_fit = nullptr;
+    double* d_dist = nullptr;
+    int32_t* d_idx = nullptr;
+    double* h_dist = nullptr;
+    int32_t* h_idx = nullptr;
+    mdsctk_knn_ctx* ctx = nullptr;
+    FILE* fd = nullptr;
+    FILE* fi = nullptr;
+    ~Held() {
+      if (ctx) mdsctk_knn_destroy(ctx);
+      mdsctk_xtc_free_device(d_ref);
+      mdsctk_xtc_free_device(d_fit);
+      (void)hipFree(d_dist);
+      (void)hipFree(d_idx);
+      if (h_dist) (void)hipHostFree(h_dist);
+      if (h_idx) (void)hipHostFree(h_idx);
+      if (fd) std::fclose(fd);
+      if (fi) std::fclose(fi);
+    }
+  } h;
+
+  // XTC -> device (knn_rms.cpp:155-206); a natoms mismatch exits with 4
+  auto load = [&](const std::string& path, float** d, int64_t* n) -> int {
+    const int rc = mdsctk_xtc_read_device(path.c_str(), natoms, device, d, n);
+    if (rc == MDSCTK_ERR_NATOMS) {
+      int32_t na = 0;
+      mdsctk_xtc_natoms(path.c_str(), &na);
+      std::printf("*** ERROR ***\nNumber of atoms in topology file (%d) does not match the number "
+                  "of atoms in the XTC file (%s : %d).\n", natoms, path.c_str(), na);
+      return 4;
+    }
+    if (rc < 0) {
+      std::printf("\nERROR: %s\n", mdsctk_io_last_error());
+      return 3;
+    }
+    return 0;
+  };
+
+  // Both files' atom counts are checked at open, before any frame is read
+  // (knn_rms.cpp:155-178).
+  for (const std::string* path : {&reff, &fitf}) {
     int32_t na = 0;
-    float pr = 0;
-    if (mdsctk_xtc_info(path.c_str(), &nf, &na, &pr) < 0) {
+    if (mdsctk_xtc_natoms(path->c_str(), &na) < 0) {
       std::printf("\nERROR: %s\n", mdsctk_io_last_error());
       return 3;
     }
     if (na != natoms) {
       std::printf("*** ERROR ***\nNumber of atoms in topology file (%d) does not match the number "
-                  "of atoms in the XTC file (%s : %d).\n", natoms, path.c_str(), na);
+                  "of atoms in the XTC file (%s : %d).\n", natoms, path->c_str(), na);
       return 4;
     }
-    xyz.resize((size_t)nf * natoms * 3);
-    int64_t got = 0;
-    if (mdsctk_xtc_read(path.c_str(), xyz.data(), nf, natoms, &got, nullptr, nullptr, nullptr,
-                        nthreads) < 0) {
-      std::printf("\nERROR: %s\n", mdsctk_io_last_error());
-      return 3;
-    }
-    *n = got;
-    return 0;
-  };
+  }
 
-  std::vector<float> ref_xyz, fit_xyz;
   int64_t n_ref = 0, n_fit = 0;
   std::printf("Reading reference coordinates from file: %s ... ", reff.c_str());
   std::fflush(stdout);
-  if (int rc = load(reff, ref_xyz, &n_ref)) return rc;
+  if (int rc = load(reff, &h.d_ref, &n_ref)) return rc;
   std::printf("done.\n");
   if (!self) {
     std::printf("Reading fitting coordinates from file: %s ... ", fitf.c_str());
     std::fflush(stdout);
-    if (int rc = load(fitf, fit_xyz, &n_fit)) return rc;
+    if (int rc = load(fitf, &h.d_fit, &n_fit)) return rc;
     std::printf("done.\n");
   } else {
     n_fit = n_ref;
   }
+  const double t_read = secs_since(t_start);
   if (n_ref == 0) {
     std::printf("ERROR: %s holds no frames.\n", reff.c_str());
     return 3;
   }
   std::printf("Block size: %d\n", std::max(1, (int)std::min<int64_t>(blksize, n_fit)));
 
-  mdsctk_knn_ctx* ctx = nullptr;
-  int rc = mdsctk_knn_create(&ctx, device, precision);
+  int rc = mdsctk_knn_create(&h.ctx, device, precision);
   if (rc < 0) {
+    h.ctx = nullptr;
     std::printf("ERROR: cannot create the GPU context on device %d (code %d)\n", device, rc);
     return 2;
   }
-  const auto t0 = std::chrono::steady_clock::now();
-  rc = mdsctk_knn_set_reference(ctx, ref_xyz.data(), n_ref, natoms, masses.data());
+  mdsctk_knn_ctx* ctx = h.ctx;
+  const auto t0 = clk::now();
+  rc = mdsctk_knn_set_reference_device(ctx, h.d_ref, n_ref, natoms, masses.data());
   if (rc < 0) {
     std::printf("ERROR: %s\n", mdsctk_knn_last_error(ctx));
-    mdsctk_knn_destroy(ctx);
     return 2;
   }
   uint32_t flags = 0;
   if (nofit) flags |= MDSCTK_KNN_NOFIT;
   if (!sort) flags |= MDSCTK_KNN_FULL_ROWS;
   const int width = mdsctk_knn_row_width(ctx, k, flags);
-  std::vector<double> dist((size_t)n_fit * std::max(width, 0));
-  std::vector<int32_t> idx((size_t)n_fit * std::max(width, 0));
-  rc = mdsctk_knn_query(ctx, self ? nullptr : fit_xyz.data(), n_fit, k, flags, dist.data(),
-                        idx.data());
-  if (rc < 0) {
+  if (width < 0) {
     std::printf("ERROR: %s\n", mdsctk_knn_last_error(ctx));
-    mdsctk_knn_destroy(ctx);
     return 2;
   }
-  mdsctk_knn_stats st;
-  mdsctk_knn_get_stats(ctx, &st);
-  mdsctk_knn_destroy(ctx);
-  const double secs =
-      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-
-  if (!write_all(df, dist.data(), dist.size() * sizeof(double)) ||
-      !write_all(inf, idx.data(), idx.size() * sizeof(int32_t))) {
+  h.fd = std::fopen(df.c_str(), "wb");
+  h.fi = std::fopen(inf.c_str(), "wb");
+  if (!h.fd || !h.fi) {
     std::printf("ERROR: cannot write %s / %s\n", df.c_str(), inf.c_str());
     return 3;
   }
+  // Row blocks of at most 2^27 output values (1 GiB of doubles).  A self
+  // query that fits one block runs as one self query (each unordered pair
+  // scored once); otherwise each block queries its frames' rows.
+  const int64_t w = std::max(width, 1);
+  const int64_t rows_per_block = std::max<int64_t>(1, std::min<int64_t>(n_fit, (int64_t(1) << 27) / w));
+  const size_t blk_vals = (size_t)rows_per_block * w;
+  if (hipMalloc((void**)&h.d_dist, blk_vals * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&h.d_idx, blk_vals * sizeof(int32_t)) != hipSuccess ||
+      hipHostMalloc((void**)&h.h_dist, blk_vals * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&h.h_idx, blk_vals * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+    std::printf("ERROR: cannot allocate the output blocks (%zu values)\n", blk_vals);
+    return 2;
+  }
+  const float* fit_frames = self ? h.d_ref : h.d_fit;
+  const bool one_self_query = self && rows_per_block >= n_fit;
+  double gpu_ms = 0;
+  int64_t n_cand = 0, n_exact = 0;
+  double t_write = 0;
+  for (int64_t r0 = 0; r0 < n_fit; r0 += rows_per_block) {
+    const int64_t nr = std::min(rows_per_block, n_fit - r0);
+    rc = mdsctk_knn_query_device(ctx, one_self_query ? nullptr : fit_frames + r0 * natoms * 3, nr,
+                                 k, flags, h.d_dist, h.d_idx);
+    if (rc < 0) {
+      std::printf("ERROR: %s\n", mdsctk_knn_last_error(ctx));
+      return 2;
+    }
+    mdsctk_knn_stats st;
+    mdsctk_knn_get_stats(ctx, &st);
+    gpu_ms += st.ms_total;
+    n_cand += st.n_candidates;
+    n_exact += st.n_fallback_rows;
+    const size_t nv = (size_t)nr * width;
+    if (hipMemcpy(h.h_dist, h.d_dist, nv * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(h.h_idx, h.d_idx, nv * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+      std::printf("ERROR: cannot copy the outputs back\n");
+      return 2;
+    }
+    const auto tw = clk::now();
+    if (std::fwrite(h.h_dist, sizeof(double), nv, h.fd) != nv ||
+        std::fwrite(h.h_idx, sizeof(int32_t), nv, h.fi) != nv) {
+      std::printf("ERROR: cannot write %s / %s\n", df.c_str(), inf.c_str());
+      return 3;
+    }
+    t_write += secs_since(tw);
+  }
+  if (std::fclose(h.fd) != 0 || std::fclose(h.fi) != 0) {
+    h.fd = h.fi = nullptr;
+    std::printf("ERROR: cannot write %s / %s\n", df.c_str(), inf.c_str());
+    return 3;
+  }
+  h.fd = h.fi = nullptr;
+  const double secs = secs_since(t0);
+  const double pairs = (double)n_fit * (double)n_ref;
   std::printf("\n%lld x %lld pairs, k=%d, %s: %.3f s (GPU %.3f ms, %.3g pairs/s), "
-              "%lld candidates re-scored, %lld exact rows\n\n",
-              (long long)n_fit, (long long)n_ref, width, prec.c_str(), secs, st.ms_total,
-              st.ms_total > 0 ? (double)st.n_pairs / (st.ms_total * 1e-3) : 0.0,
-              (long long)st.n_candidates, (long long)st.n_fallback_rows);
+              "%lld candidates re-scored, %lld exact rows; XTC read and decode %.3f s, "
+              "file writes %.3f s, wall %.3f s\n\n",
+              (long long)n_fit, (long long)n_ref, width, prec.c_str(), secs, gpu_ms,
+              gpu_ms > 0 ? pairs / (gpu_ms * 1e-3) : 0.0, (long long)n_cand, (long long)n_exact,
+              t_read, t_write, secs_since(t_start));
   return 0;
 }

@@ -68,6 +68,11 @@ struct ServeSlot {
   size_t h_elems = 0, h_rows = 0;
   hipEvent_t done = nullptr;
   hipGraphExec_t exec = nullptr;  // the slot's batch kernels, updated per batch
+  // the captured graph exec was last instantiated or updated from; kept
+  // until a later capture has replaced it in exec (and that exec has been
+  // launched), so nothing the executable may refer to is released while it
+  // can still run
+  hipGraph_t graph = nullptr;
   int64_t limit = 0;  // frame limit of the batch in the slot
   // shape the workspace was last sized for
   int64_t sized_limit = 0;
@@ -110,6 +115,7 @@ class Server {
     for (auto& sl : slots_) {
       if (sl.s.st) (void)hipStreamSynchronize(sl.s.st);
       if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+      if (sl.graph) (void)hipGraphDestroy(sl.graph);
       slot_release(c_, sl.s);
       if (sl.done) (void)hipEventDestroy(sl.done);
       free_host(sl);
@@ -372,6 +378,10 @@ class Server {
   // topology, new sizes and launch shapes); it is rebuilt only when the
   // update is refused.  Nothing in the sequence is a memset or an
   // allocation: plan_fused has sized the workspace before the capture.
+  // The captured graph stays alive until the next capture has replaced it:
+  // the executable is only ever launched while the graph it came from
+  // exists (a tracer reading the launched kernels' node data at launch time
+  // then never reads released memory).
   int replay_graph(ServeSlot& sl, const FusedPlan& plan) {
     Slot& s = sl.s;
     hipError_t e = hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal);
@@ -403,8 +413,12 @@ class Server {
       }
       ++graph_builds_;
     }
-    (void)hipGraphDestroy(g);
     e = hipGraphLaunch(sl.exec, s.st);
+    // exec now holds g's parameters and has been launched: the graph it
+    // replaced can go (the slot's stream orders every earlier launch of
+    // this exec before this one)
+    if (sl.graph) (void)hipGraphDestroy(sl.graph);
+    sl.graph = g;
     if (e != hipSuccess) return hip_fail(c_, e, "serve: graph launch");
     ++graph_launches_;
     return 0;

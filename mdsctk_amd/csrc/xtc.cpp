@@ -543,6 +543,19 @@ int mdsctk_xtc_read(const char* path, float* xyz, int64_t max_frames, int32_t na
   return 0;
 }
 
+int mdsctk_xtc_natoms(const char* path, int32_t* natoms) {
+  if (!path || !natoms) return io_fail(MDSCTK_ERR_ARG, "null argument%s%lld", "", 0);
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return io_fail(MDSCTK_ERR_IO, "cannot open %s%lld", path, 0);
+  unsigned char h0[8];
+  const size_t got = std::fread(h0, 1, sizeof h0, f);
+  std::fclose(f);
+  if (got != sizeof h0 || be_i32(h0) != kMagic)
+    return io_fail(MDSCTK_ERR_FORMAT, "%s does not start with an XTC frame%lld", path, 0);
+  *natoms = be_i32(h0 + 4);
+  return 0;
+}
+
 int mdsctk_xtc_write(const char* path, const float* xyz, int64_t n_frames, int32_t natoms,
                      float precision, const int32_t* steps, const float* times,
                      const float* boxes) {

@@ -35,6 +35,7 @@ OPT_SERVE_SLOTS = 4
 OPT_SERVE_GRAPHS = 6
 OPT_SERVE_SHARE_CUS = 7
 OPT_SELF_SYMMETRIC = 8
+OPT_MIRROR_CAP = 9
 OPT_SERVE_MAX_FRAMES = 5
 
 ERR_ARG = -1
@@ -61,6 +62,11 @@ class Stats(ctypes.Structure):
         ("list_capacity", ctypes.c_int32),
         ("eps_rel", ctypes.c_double),
         ("n_pairs_computed", ctypes.c_int64),
+        ("ms_landmark", ctypes.c_double),
+        ("n_pairs_landmark", ctypes.c_int64),
+        ("n_mirrored", ctypes.c_int64),
+        ("n_mirrored_max", ctypes.c_int64),
+        ("n_relisted_rows", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -175,6 +181,13 @@ def _bind_io(L):
         L.mdsctk_xtc_write.restype = ctypes.c_int
         L.mdsctk_topology_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32), vp, i32]
         L.mdsctk_topology_read.restype = ctypes.c_int
+        L.mdsctk_xtc_read_device.argtypes = [ctypes.c_char_p, i32, ctypes.c_int, ctypes.POINTER(vp),
+                                             ctypes.POINTER(i64)]
+        L.mdsctk_xtc_read_device.restype = ctypes.c_int
+        L.mdsctk_xtc_free_device.argtypes = [vp]
+        L.mdsctk_xtc_free_device.restype = ctypes.c_int
+        L.mdsctk_xtc_natoms.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]
+        L.mdsctk_xtc_natoms.restype = ctypes.c_int
         L.mdsctk_io_last_error.argtypes = []
         L.mdsctk_io_last_error.restype = ctypes.c_char_p
 

@@ -48,9 +48,9 @@ def test_knn_rms_self_query_files(tmp_path, trp, prec):
 
 
 def test_knn_rms_self_query_forms_agree(tmp_path, trp):
-    """-f omitted, -f equal to -r (both a self query through the score
-    matrix) and -f naming a copy of the file (a cross query through the
-    candidate lists) write identical files."""
+    """-f omitted, -f equal to -r (both a self query scoring each
+    unordered pair once) and -f naming a copy of the file (a cross query
+    through the candidate lists) write identical files."""
     import shutil
     copy = tmp_path / "copy.xtc"
     shutil.copy(TRP_XTC, copy)
@@ -98,3 +98,36 @@ def test_example_pipeline_spectrum(tmp_path, trp):
     n, pcol, irow, vals = og.ssm_build(d_o, i_o)
     w_o, _ = og.spectral_decomp(n, pcol, irow, vals, float(np.median(d_o)), 8)
     assert np.abs(ev - w_o).max() < 1e-6
+
+
+def test_knn_rms_full_rows_stream_in_blocks(tmp_path):
+    """--sort false with more than 2^27 output values: the rows are produced
+    and written in two blocks (host memory stays bounded); sampled rows on
+    either side of the block boundary equal the library's full rows, and
+    every index row is the identity."""
+    from mdsctk_amd import knn, synth
+    from tests.test_formats import xtc_write
+    n, natoms = 11600, 20  # 1.3456e8 values > 2^27
+    x = synth.trajectory(n, natoms, 44)
+    f = str(tmp_path / "t.xtc")
+    assert xtc_write(f, x, 1e5) == 0
+    pdb = tmp_path / "t.pdb"
+    with open(pdb, "w") as fh:
+        for a in range(natoms):
+            fh.write("ATOM  %5d  CA  ALA A%4d    %8.3f%8.3f%8.3f  1.00  0.00           C\n"
+                     % (a + 1, a + 1, 0.0, 0.0, 0.0))
+    p = run(build.CLI_PATH, ["-s", "false", "-p", str(pdb), "-r", f], tmp_path)
+    assert p.returncode == 0, p.stdout + p.stderr
+    d = np.memmap(tmp_path / "distances.dat", np.float64, "r", shape=(n, n))
+    i = np.memmap(tmp_path / "indices.dat", np.int32, "r", shape=(n, n))
+    rows = [0, 1, 11566, 11567, 11568, 11599]
+    from tests.test_formats import topology, xtc_read
+    _, xd, *_ = xtc_read(f)
+    rc, w = topology(str(pdb))
+    assert rc >= 0 and w.shape == (natoms,)
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_reference(xd, w)
+        d_l, _ = ctx.query(xd[rows], 0, knn.KNN_FULL_ROWS)
+    np.testing.assert_array_equal(np.asarray(d[rows]), d_l)
+    for r in rows:
+        assert (np.asarray(i[r]) == np.arange(n)).all()

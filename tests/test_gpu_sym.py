@@ -1,6 +1,7 @@
-"""GPU parity of the symmetric self query: the contraction scores the upper
-block triangle into an n x n FP32 matrix, a select pass picks each row's
-candidates, and the merge re-scores them.  Its outputs must equal the
+"""GPU parity of the symmetric self query: a landmark pass bounds every
+row's threshold, the contraction scores the upper block triangle once into
+each fit frame's candidate lists and each reference frame's mirrored list
+(no score matrix), and the merge re-scores both.  Its outputs must equal the
 candidate-list path's bit for bit (both re-score a certified window in
 FP64), and the FP64 oracle's within the distance tolerance.
 
@@ -15,11 +16,13 @@ from tests.parity import check_knn
 pytestmark = pytest.mark.gpu
 
 
-def _self_query(prec, frames, w, k, flags=0, sym=True, budget=0):
+def _self_query(prec, frames, w, k, flags=0, sym=True, budget=0, mirror_cap=0):
     with knn.KnnContext(0, prec) as ctx:
         ctx.set_option(knn.OPT_SELF_SYMMETRIC, 1 if sym else 0)
         if budget:
             ctx.set_option(knn.OPT_HBM_BUDGET, budget)
+        if mirror_cap:
+            ctx.set_option(knn.OPT_MIRROR_CAP, mirror_cap)
         ctx.set_reference(frames, w)
         d, i = ctx.query(None, k, flags)
         return d, i, ctx.stats()
@@ -36,6 +39,8 @@ def _both(prec, frames, w, k, flags=0):
     d_l, i_l, st_l = _self_query(prec, frames, w, k, flags, sym=False)
     n = len(frames)
     print(prec, n, k, flags, "sym pairs", st_s["n_pairs_computed"], "of", st_s["n_pairs"],
+          "landmark", st_s["n_pairs_landmark"], "mirrored", st_s["n_mirrored"], "max",
+          st_s["n_mirrored_max"], "relisted", st_s["n_relisted_rows"],
           "cand", st_s["n_candidates"], "vs", st_l["n_candidates"],
           "fallback", st_s["n_fallback_rows"], st_l["n_fallback_rows"])
     assert st_l["n_pairs_computed"] == n * n
@@ -75,8 +80,9 @@ def test_symmetric_vs_oracle(prec):
 
 def test_symmetric_duplicates_overflow_to_exact_rows():
     """More than the list capacity of equal (zero) keys in every row: the
-    select pass overflows, the merge sends the rows to the exact path, and
-    the outputs still equal the list path's (ties by index)."""
+    candidate lists overflow, the merge sends the rows through the lists
+    again and then to the exact path, and the outputs still equal the list
+    path's (ties by index)."""
     rng = np.random.default_rng(1)
     base = rng.normal(size=(1, 40, 3)).astype(np.float32)
     frames = np.repeat(base, 2200, axis=0)
@@ -88,10 +94,10 @@ def test_symmetric_duplicates_overflow_to_exact_rows():
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp32"])
-def test_symmetric_duplicates_beyond_stage(prec):
-    """5000 equal frames: more keys than the select pass can stage in LDS
-    fall near the (k+1)-th smallest, so it reads the row again for the fine
-    histogram and the gather; outputs still equal the list path's."""
+def test_symmetric_many_duplicates(prec):
+    """5000 equal frames: every one of them lies below every equal frame's
+    threshold, so their mirrored lists fill with thousands of zero keys;
+    outputs still equal the list path's."""
     rng = np.random.default_rng(2)
     base = rng.normal(size=(1, 40, 3)).astype(np.float32)
     frames = np.repeat(base, 5100, axis=0)
@@ -101,17 +107,17 @@ def test_symmetric_duplicates_beyond_stage(prec):
     assert np.abs(d[:5000]).max() < 1e-5
 
 
-def test_symmetric_sampled_select():
-    """Rows of 32768 keys or more take the sampled select.  34000 distinct
-    frames (the sampled bound holds and stages few keys) plus 6000 equal
-    ones (their near-zero keys overflow the LDS stage, so those rows fall
-    back to the full select and then to exact rows)."""
+def test_symmetric_large_set_with_duplicates():
+    """34000 distinct frames (the landmark bound keeps their mirrored lists
+    short) plus 6000 equal ones (their zero keys overflow the candidate
+    lists, so those rows end on the exact path)."""
     frames = synth.trajectory(40000, 16, 21)
     frames[34000:] = frames[34000]
     w = synth.masses(16, "uniform")
     d, i, st = _both("bf16", frames, w, 7)
     assert st["n_fallback_rows"] >= 6000
     assert np.abs(d[34000:]).max() < 1e-5
+    assert st["n_mirrored_max"] <= 8192
 
 
 def test_symmetric_large_k():
@@ -121,8 +127,9 @@ def test_symmetric_large_k():
 
 
 def test_symmetric_budget_falls_back_to_lists():
-    """A budget that holds the candidate lists (about 1.3 GB here) but not
-    the 1.6 GB score matrix keeps the list path, with the same outputs."""
+    """A budget that holds the list path's candidate lists (about 1.3 GB
+    here) but not the symmetric form's lists and mirrored lists (about
+    2.0 GB) keeps the list path, with the same outputs."""
     n = 20000
     frames = synth.trajectory(n, 16, 4)
     w = synth.masses(16, "uniform")
@@ -152,10 +159,10 @@ def test_symmetric_data_vectors(metric):
 
 
 def test_symmetric_then_cross_then_serving_on_one_context():
-    """A self query (score matrix), a cross query (lists) and a served
+    """A self query (mirrored lists), a cross query (lists) and a served
     request share one context's workspace; each gives what it gives on a
-    fresh context.  While serving runs, a self query frees its score matrix
-    when it ends."""
+    fresh context.  A cross query, and a self query ending while serving
+    runs, free the mirrored lists."""
     frames = synth.metastable_trajectory(1500, 100, 8, n_centres=5, dup_fraction=0.1)
     other = synth.trajectory(400, 100, 9)
     w = synth.masses(100, "backbone")
@@ -204,25 +211,85 @@ def test_symmetric_tiny_sets(n, flags):
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp32"])
-def test_symmetric_wide_k_beyond_list_capacity(prec):
-    """k+1 above the candidate lists' 896: a self query still takes the
-    symmetric form (select list and merge hold 2048) and equals the
-    all-pairs FP64 path bit for bit."""
+def test_self_query_wide_k_takes_the_exact_path(prec):
+    """k+1 above the candidate lists' 896: a self query scores every pair
+    exactly in FP64, with or without the symmetric option, and matches the
+    oracle."""
     frames = synth.metastable_trajectory(3000, 48, 12, n_centres=4, dup_fraction=0.05)
     w = synth.masses(48, "backbone")
     k = 1200
     d_s, i_s, st_s = _self_query(prec, frames, w, k)
     d_e, i_e, st_e = _self_query(prec, frames, w, k, sym=False)
-    print(prec, "wide k", st_s["n_pairs_computed"], st_s["n_candidates"], st_s["n_fallback_rows"],
-          st_s["ms_total"], "ms vs exact", st_e["ms_total"], "ms")
-    assert st_s["n_pairs_computed"] == _triangle_pairs(3000, 64 if prec == "fp32" else 128)
-    assert st_e["n_fallback_rows"] == 3000        # the all-pairs FP64 path
+    assert st_s["n_fallback_rows"] == 3000 and st_e["n_fallback_rows"] == 3000
     np.testing.assert_array_equal(i_s, i_e)
     np.testing.assert_array_equal(d_s, d_e)
     rows = np.arange(0, 3000, 97)
     d_o, i_o = oracle.knn_exact(frames, frames[rows], w, k)
     # rank 0 (the frame itself) is dropped in both
     check_knn(frames, frames[rows], w, d_s[rows], i_s[rows], d_o, i_o)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+@pytest.mark.parametrize("cap", [1, 40])
+def test_symmetric_mirror_overflow_relists_rows(prec, cap):
+    """Mirrored lists too short for what the rows receive: the merge flags
+    those rows, they are computed again through the candidate lists, and the
+    outputs equal the list path's."""
+    frames = synth.metastable_trajectory(2500, 80, 19, n_centres=3, dup_fraction=0.05)
+    w = synth.masses(80, "backbone")
+    k = 12
+    d_s, i_s, st_s = _self_query(prec, frames, w, k, mirror_cap=cap)
+    d_l, i_l, st_l = _self_query(prec, frames, w, k, sym=False)
+    print(prec, cap, "relisted", st_s["n_relisted_rows"], "exact", st_s["n_fallback_rows"],
+          "mirrored", st_s["n_mirrored"], "max", st_s["n_mirrored_max"])
+    assert st_s["n_pairs_computed"] < 2500 * 2500
+    assert st_s["n_relisted_rows"] > 0
+    np.testing.assert_array_equal(i_s, i_l)
+    np.testing.assert_array_equal(d_s, d_l)
+
+
+def test_symmetric_then_cross_then_serving_within_a_budget():
+    """A budget that holds the symmetric self query's lists, but not those
+    plus what a later cross query and serving need unless the mirrored lists
+    are freed: self, cross and served queries all succeed, with the outputs
+    of fresh contexts."""
+    n = 20000
+    frames = synth.trajectory(n, 16, 4)
+    other = synth.trajectory(3000, 16, 5, base_seed=4)
+    w = synth.masses(16, "uniform")
+    d_s, i_s, st_s = _self_query("bf16", frames, w, 10)
+    assert st_s["n_pairs_computed"] < n * n
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_reference(frames, w)
+        d_c, i_c = ctx.query(other, 10)
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_option(knn.OPT_HBM_BUDGET, 2.3e9)
+        ctx.set_reference(frames, w)
+        d, i = ctx.query(None, 10)
+        assert ctx.stats()["n_pairs_computed"] < n * n
+        np.testing.assert_array_equal(d, d_s)
+        np.testing.assert_array_equal(i, i_s)
+        d, i = ctx.query(other, 10)
+        np.testing.assert_array_equal(d, d_c)
+        np.testing.assert_array_equal(i, i_c)
+        r = ctx.wait(ctx.submit(other, 10))
+        np.testing.assert_array_equal(r.dist, d_c)
+        np.testing.assert_array_equal(r.idx, i_c)
+
+
+def test_symmetric_400k_frames():
+    """400,000 frames (a size whose n x n FP32 matrix, 640 GB, would not fit
+    one GPU) take the symmetric form: lists only."""
+    n, natoms, k = 400_000, 24, 16
+    frames = synth.trajectory(n, natoms, 40)
+    w = synth.masses(natoms, "uniform")
+    d, i, st = _self_query("bf16", frames, w, k)
+    print("400k self query", st)
+    assert st["n_pairs_computed"] < n * n // 2 + n * 256
+    assert st["n_fallback_rows"] == 0
+    rows = np.array([0, 1234, 199_999, 399_999])
+    d_o, i_o = oracle.knn_exact(frames, frames[rows], w, k)
+    check_knn(frames, frames[rows], w, d[rows], i[rows], d_o, i_o)
 
 
 @pytest.mark.parametrize("sym", [1, 0])
