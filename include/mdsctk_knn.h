@@ -160,6 +160,24 @@ int mdsctk_knn_query(mdsctk_knn_ctx* ctx, const float* xyz_nm, int64_t n_fit, in
 int mdsctk_knn_query_device(mdsctk_knn_ctx* ctx, const float* d_xyz_nm, int64_t n_fit,
                             int32_t k, uint32_t flags, double* d_out_dist, int32_t* d_out_idx);
 
+/* Same as mdsctk_knn_query_device, but the call allocates nothing: no
+ * hipMalloc or hipFree, and no other allocating routine.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * it).  Its workspace is a fixed per-GPU region of 128 MiB inside the
+ * library (held by every GPU the library runs on until the process ends).
+ * Sizes whose workspace does not fit it are refused with MDSCTK_ERR_ARG:
+ * the query frames' derived arrays (about 12 natoms + 64 bytes each), their
+ * candidate lists (the reference phases are lowered until they fit), and
+ * one exact FP64 row of 8 n_ref bytes.  A self query (d_xyz_nm NULL) takes
+ * the candidate lists, never the score matrix.  Frames only (a context
+ * holding data vectors is refused).  Every call on one GPU shares that
+ * GPU's region, whatever the context: calls that overlap on one GPU race,
+ * so callers serialise them. */
+int mdsctk_knn_query_device_noalloc(mdsctk_knn_ctx* ctx, const float* d_xyz_nm, int64_t n_fit,
+                                    int32_t k, uint32_t flags, double* d_out_dist,
+                                    int32_t* d_out_idx);
+
 /* Diagnostic (no counterpart in the reference): the approximate score
  * s~ = G_r + G_f - 2 lambda~ (mass nm^2) that the contraction form computes
  * for every pair, and optionally the exact FP64 score, both [n_fit][n_ref].

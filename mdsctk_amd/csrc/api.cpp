@@ -1028,6 +1028,15 @@ int mdsctk_knn_create(mdsctk_knn_ctx** out, int device, int precision) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     c->n_cu = prop.multiProcessorCount;
+  {
+    // the allocation-free query's fixed region on this device (its address
+    // is looked up once per device, with the device current)
+    const hipError_t e = scratch_region(device, &c->scratch);
+    if (e != hipSuccess) {
+      c->scratch = nullptr;
+      c->scratch_err = hipGetErrorString(e);
+    }
+  }
   const int rc = slot_init(c, c->main);
   if (rc) {
     mdsctk_knn_destroy(c);
@@ -1123,6 +1132,22 @@ int mdsctk_knn_query_device(mdsctk_knn_ctx* c, const float* d_xyz_nm, int64_t n_
   return query_impl(c, d_xyz_nm, false, true, n_fit, k, flags, d_out_dist, d_out_idx, true);
 }
 
+
+int mdsctk_knn_query_device_noalloc(mdsctk_knn_ctx* c, const float* d_xyz_nm, int64_t n_fit,
+                                    int32_t k, uint32_t flags, double* d_out_dist,
+                                    int32_t* d_out_idx) {
+  if (!c) return MDSCTK_ERR_ARG;
+  if (!d_out_dist || !d_out_idx) return fail(c, MDSCTK_ERR_ARG, "query (fixed workspace): null output");
+  if (c->has_ref && c->w.metric != kMetricRMSD)
+    return fail(c, MDSCTK_ERR_STATE, "query (fixed workspace): the context holds data vectors");
+  if (!c->scratch)
+    return fail(c, MDSCTK_ERR_HIP, "query (fixed workspace): the fixed region is unavailable on device %d: %s",
+                c->device, c->scratch_err.c_str());
+  Carve cv;
+  cv.base = (char*)c->scratch;
+  cv.size = kScratchBytes;
+  return query_impl(c, d_xyz_nm, false, true, n_fit, k, flags, d_out_dist, d_out_idx, true, &cv);
+}
 
 int mdsctk_knn_debug_scores(mdsctk_knn_ctx* c, const float* xyz_nm, int64_t n_fit,
                             uint32_t flags, double* approx_s, double* exact_s) {

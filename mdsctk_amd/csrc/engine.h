@@ -130,6 +130,10 @@ struct mdsctk_knn_ctx {
   mdsctk_knn_stats stats = {};
   mdsctk::Server* server = nullptr;
   std::mutex server_mu;  // guards creating / stopping the serving thread
+  // This device's fixed workspace of the allocation-free query (looked up
+  // at create; null with the reason in scratch_err when unavailable).
+  void* scratch = nullptr;
+  std::string scratch_err;
 };
 
 namespace mdsctk {

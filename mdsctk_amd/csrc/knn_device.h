@@ -195,6 +195,13 @@ hipError_t launch_column_mean(const double* x, int64_t n, int vdim, double* mean
 hipError_t launch_data_prep(const DevFrames& f, int metric, const double* shift, hipStream_t st);
 hipError_t launch_max_gt(const DevFrames& f, double* out, hipStream_t st);
 
+// Size of the per-GPU fixed workspace of the allocation-free query entry
+// point, and its device address on `device` (which must be the current
+// device on the first call for that device; later calls return the cached
+// address).  Calls that overlap on one GPU share it.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+hipError_t scratch_region(int device, void** addr);
+
 // Fit tiles per contraction workgroup for a precision (2 or 4).
 int contract_fit_tiles(int prec);
 // Choose the per-list capacity for k1 (0 when the fused path cannot serve it).

@@ -40,6 +40,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <mutex>
 
 #include "knn_device.h"
 
@@ -2002,5 +2003,38 @@ hipError_t launch_identity_rows(int64_t n_fit, int64_t n_ref, int* out_idx, hipS
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Fixed workspace of the allocation-free query (mdsctk_knn_query_device_noalloc):
+// a file-scope device array, one copy per GPU, held until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory, so
+// it is never passed to a kernel or a copy: scratch_region() looks up each
+// device's own address once (hipGetSymbolAddress, with that device current)
+// and every caller uses that address.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kScratchDevices] = {};
+}  // namespace
+
+hipError_t scratch_region(int device, void** addr) {
+  *addr = nullptr;
+  if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    int current = -1;
+    hipError_t e = hipGetDevice(&current);
+    if (e != hipSuccess) return e;
+    if (current != device) return hipErrorInvalidDevice;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_addr[device] = p;
+  }
+  *addr = g_scratch_addr[device];
+  return hipSuccess;
+}
 
 }  // namespace mdsctk

@@ -141,6 +141,8 @@ def load():
     L.mdsctk_knn_query.restype = ctypes.c_int
     L.mdsctk_knn_query_device.argtypes = [vp, vp, i64, i32, u32, vp, vp]
     L.mdsctk_knn_query_device.restype = ctypes.c_int
+    L.mdsctk_knn_query_device_noalloc.argtypes = [vp, vp, i64, i32, u32, vp, vp]
+    L.mdsctk_knn_query_device_noalloc.restype = ctypes.c_int
     L.mdsctk_knn_debug_scores.argtypes = [vp, vp, i64, u32, vp, vp]
     L.mdsctk_knn_debug_scores.restype = ctypes.c_int
     L.mdsctk_knn_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
@@ -280,6 +282,13 @@ class KnnContext:
     def query_device(self, d_xyz, n_fit, k, flags, d_dist, d_idx):
         """Device-pointer query (ints are raw device addresses; d_xyz 0 = self)."""
         return self._check(self._L.mdsctk_knn_query_device(
+            self._h, ctypes.c_void_p(d_xyz) if d_xyz else None, n_fit, k, flags,
+            ctypes.c_void_p(d_dist), ctypes.c_void_p(d_idx)))
+
+    def query_device_noalloc(self, d_xyz, n_fit, k, flags, d_dist, d_idx):
+        """As query_device, through the entry point that allocates nothing
+        (its workspace is the library's fixed per-GPU region)."""
+        return self._check(self._L.mdsctk_knn_query_device_noalloc(
             self._h, ctypes.c_void_p(d_xyz) if d_xyz else None, n_fit, k, flags,
             ctypes.c_void_p(d_dist), ctypes.c_void_p(d_idx)))
 
