@@ -98,9 +98,12 @@ def time_fused(ctx, hip, n_fit, k, steps, warmup, first=None):
         for s in range(steps):
             ctx.query_device(0, n_fit, k, 0, d_dist, d_idx)
             stats.append(ctx.stats())
-            log(f"  step {s}: {stats[-1]['ms_total']:.1f} ms device, contraction "
-                f"{stats[-1]['ms_contract']:.1f} ms, merge {stats[-1]['ms_merge']:.1f} ms, "
-                f"fallback rows {stats[-1]['n_fallback_rows']}")
+            st = stats[-1]
+            log(f"  step {s}: {st['ms_total']:.1f} ms device, landmark {st['ms_landmark']:.1f} ms, "
+                f"contraction {st['ms_contract']:.1f} ms, merge {st['ms_merge']:.1f} ms, "
+                f"relisted rows {st['n_relisted_rows']} ({st['ms_fallback']:.1f} ms), "
+                f"fallback rows {st['n_fallback_rows']}, mirrored {st['n_mirrored']} "
+                f"(max {st['n_mirrored_max']} per row)")
         hip.sync()
         wall = (time.perf_counter() - t0) / steps
         dist = np.zeros((n_fit, k), np.float64)
@@ -303,6 +306,60 @@ def run_mixed_sets(knn, synth, oracle, args):
                             "gpu_knn_s": gpu_s, "cpu_knn_s": cpu_s}}
 
 
+def run_cli_e2e(knn, synth, args):
+    """The drop-in executable end to end on the precision-form workload: a
+    200,000-frame x 1000-atom XTC at precision 1e5 (1e-5 nm steps keep the
+    1e-4 nm near-duplicates) and a 1000-atom PDB are written to a scratch
+    folder; knn_rms -k 64 runs as a child process (wall time from its start
+    to its exit: topology, XTC read and GPU decode, GPU context, query,
+    output files); its files must equal the library's query of the decoded
+    frames bit for bit.  The XTC was just written, so it is read from the
+    page cache."""
+    import shutil
+    import subprocess
+    import tempfile
+    from mdsctk_amd import build
+    L = knn.load()
+    work = tempfile.mkdtemp(prefix="mdsctk_cli_", dir="/tmp")
+    try:
+        n, natoms, k = args.n_frames, args.natoms, args.k
+        x = synth.metastable_trajectory(n, natoms, args.seed)
+        xtc = os.path.join(work, "traj.xtc")
+        t = time.time()
+        assert L.mdsctk_xtc_write(xtc.encode(), x.ctypes.data, n, natoms, 1e5, None, None, None) == 0
+        log(f"cli: wrote {os.path.getsize(xtc) / 1e9:.2f} GB of XTC in {time.time() - t:.1f} s")
+        del x
+        pdb = os.path.join(work, "top.pdb")
+        with open(pdb, "w") as fh:
+            for a in range(natoms):
+                fh.write("ATOM  %5d  CA  ALA A%4d    %8.3f%8.3f%8.3f  1.00  0.00           C\n"
+                         % (a + 1, a % 10000, 0.0, 0.0, 0.0))
+        cmd = [build.CLI_PATH, "-k", str(k), "-p", pdb, "-r", xtc]
+        t = time.perf_counter()
+        p = subprocess.run(cmd, cwd=work, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        wall = time.perf_counter() - t
+        if p.returncode != 0:
+            return {"error": p.stdout[-2000:]}
+        summary = [l for l in p.stdout.splitlines() if "pairs, k=" in l]
+        # the library's query of the same decoded frames
+        n_got = ctypes.c_int64()
+        xd = np.zeros((n, natoms, 3), np.float32)
+        assert L.mdsctk_xtc_read(xtc.encode(), xd.ctypes.data, n, natoms, ctypes.byref(n_got),
+                                 None, None, None, 0) == 0
+        w = np.full(natoms, 12.011, np.float32)
+        with knn.KnnContext(0, "bf16") as ctx:
+            ctx.set_reference(xd, w)
+            d, i = ctx.query(None, k)
+        same = (open(os.path.join(work, "distances.dat"), "rb").read() == d.tobytes() and
+                open(os.path.join(work, "indices.dat"), "rb").read() == i.tobytes())
+        return {"command": "knn_rms -k %d -p top.pdb -r traj.xtc" % k, "frames": n, "natoms": natoms,
+                "xtc_bytes": os.path.getsize(xtc), "wall_s": wall,
+                "files_equal_library_query": bool(same),
+                "summary": summary[-1] if summary else p.stdout[-400:]}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 COPY_PROBE_BYTES = 1 << 30
 
 
@@ -415,6 +472,9 @@ def main():
                          "many seconds (0 = skip; the CPU side of the graph check, the FP64 "
                          "oracle kNN of 30,000 frames, takes about 40 s on 16 threads)")
     ap.add_argument("--mixed-rate", type=float, default=100.0, help="mixed workload arrivals per second")
+    ap.add_argument("--cli", type=int, default=0,
+                    help="also time the drop-in knn_rms executable end to end on the workload "
+                         "(writes a 1.6 GB XTC to /tmp first; 0 = skip)")
     ap.add_argument("--cpu-rows", type=int, default=128, help="fit rows of the CPU baseline sample (0 = skip)")
     ap.add_argument("--check-rows", type=int, default=8, help="rows checked against the FP64 oracle")
     ap.add_argument("--seed", type=int, default=2026)
@@ -502,6 +562,13 @@ def main():
     out["stages_ms"] = {s: mean_stat(stats, "ms_" + s) for s in ("prep", "contract", "merge", "fallback", "total")}
     out["candidates_per_query"] = mean_stat(stats, "n_candidates") / n
     out["fallback_rows"] = int(max(s["n_fallback_rows"] for s in stats))
+    if computed < pairs:
+        # the symmetric self query's landmark pass and mirrored lists
+        out["symmetric"] = {"landmark_ms": mean_stat(stats, "ms_landmark"),
+                            "landmark_pairs": mean_stat(stats, "n_pairs_landmark"),
+                            "mirrored_per_row": mean_stat(stats, "n_mirrored") / n,
+                            "mirrored_max_row": int(max(s["n_mirrored_max"] for s in stats)),
+                            "relisted_rows": int(max(s["n_relisted_rows"] for s in stats))}
 
     # accuracy vs the FP64 oracle: 10k output pairs and a few full rows
     rng = np.random.default_rng(args.seed)
@@ -567,6 +634,9 @@ def main():
 
     if args.mixed_seconds > 0:
         out["mixed_sets"] = run_mixed_sets(knn, synth, oracle, args)
+
+    if args.cli:
+        out["cli_end_to_end"] = run_cli_e2e(knn, synth, args)
 
     if args.cpu_rows > 0:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -607,10 +607,20 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
   int phases = fused_phases(c, fit.n_tiles);
   if (c->forced_phases <= 0) phases = std::max(1, std::min(ref_pairs, std::max(4, phases / 2)));
   const int nl = phases * 4;
-  // Landmark pass: every lstride-th reference tile pair, about 4 (k+1)
-  // frames or more (stride at most kLandmarkStride, 1/64 of the set).
-  const int lstride =
-      std::max(1, std::min<int>(kLandmarkStride, (int)((int64_t)ref_pairs * 2 * kTile / (4 * q.k1))));
+  // Landmark pass: every lstride-th reference tile pair.  The (k+1)-th
+  // smallest key of L landmarks ranks about (k+1) n / L among a row's n
+  // keys, and a row may receive up to that many mirrored scores before its
+  // own lists tighten its threshold: L >= 2 (k+1) n / cap keeps that at half
+  // the mirrored capacity (3200 frames, 3 % of the pairs, at 200k frames
+  // and k = 64).  At least 4 (k+1) landmarks; stride kLandmarkStride at
+  // most, 8 at least (the pass then costs at most a quarter of the
+  // symmetric contraction, and overflowing rows are recomputed).
+  const int64_t lm_target = std::max<int64_t>(
+      4 * (int64_t)q.k1, 2 * (int64_t)q.k1 * n / std::max(1, c->mirror_cap));
+  const int lstride = std::max(
+      1, std::min<int>(kLandmarkStride,
+                       std::max<int>(std::min(8, ref_pairs),
+                                     (int)((int64_t)ref_pairs * 2 * kTile / lm_target))));
   const int lm_pairs = (ref_pairs + lstride - 1) / lstride;
   const int nlab = lm_pairs * 2 * kTile;
   const size_t ent_bytes = (size_t)n * nl * cap * sizeof(uint2);

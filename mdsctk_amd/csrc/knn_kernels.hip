@@ -873,7 +873,14 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
       // only this way round, so its scores also go to the reference frames'
       // lists.
       bool mirror = false;
-      if constexpr (MODE == kOutSym) mirror = p >= (fb + 1) * FT / kRefTilesPerWG;
+      if constexpr (MODE == kOutSym) {
+        // opaque to the optimiser: the condition is monotone in t, and
+        // splitting the tile loop at its switch would duplicate the whole
+        // K-loop and epilogue (instruction-cache pressure)
+        int mir = p >= (fb + 1) * FT / kRefTilesPerWG;
+        asm volatile("" : "+v"(mir));
+        mirror = __builtin_amdgcn_readfirstlane(mir) != 0;
+      }
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int rl = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
