@@ -25,9 +25,12 @@
  *     never calls exit().
  *   - A context is used from one host thread at a time, except for the
  *     serving calls (submit / poll / wait / serve_stats), which any number
- *     of host threads may make concurrently.  A context owns its HIP streams
- *     and all device memory it allocates.  Several contexts may exist at
- *     once (e.g. one per resident reference set).
+ *     of host threads may make concurrently, also while a synchronous query
+ *     runs on another thread.  mdsctk_knn_last_error() reports the calling
+ *     thread's last failure on the context, so concurrent calls never share
+ *     an error text.  A context owns its HIP streams and all device memory
+ *     it allocates.  Several contexts may exist at once (e.g. one per
+ *     resident reference set).
  */
 #ifndef MDSCTK_KNN_H
 #define MDSCTK_KNN_H
@@ -50,8 +53,14 @@ extern "C" {
 #define MDSCTK_ERR_PENDING (-8)  /* request still running (mdsctk_knn_poll) */
 
 /* Precision of the cross-covariance contraction.  Every form re-scores its
- * candidate shortlist in FP64, so the returned distances are FP64-exact;
- * the form only changes how candidates are found (and the speed). */
+ * candidate shortlist in FP64, so each returned distance is the FP64
+ * distance of its returned index.  The form decides how candidates are found
+ * (and the speed): the index set is exact within the candidate window's
+ * envelope, a measured bound (2x margin) on the approximate scores'
+ * deviation that the merge checks at run time on the re-scored candidates
+ * and the keys just above each row's cut (rows where it fails are recomputed
+ * exactly; a detector, not a proof).  MDSCTK_PREC_EXACT_FP64 scores every
+ * pair in FP64. */
 #define MDSCTK_PREC_FP32 0        /* FP32 MFMA (v_mfma_f32_32x32x2_f32) */
 #define MDSCTK_PREC_SPLIT_BF16 1  /* hi/lo BF16, 3 MFMA products, FP32 accumulate */
 #define MDSCTK_PREC_SPLIT_FP16 2  /* hi/lo FP16, 3 MFMA products, FP32 accumulate */
@@ -65,9 +74,11 @@ extern "C" {
 /* Options for mdsctk_knn_set_option. */
 #define MDSCTK_OPT_HBM_BUDGET 1   /* bytes of device memory the context may hold (0 = no limit) */
 #define MDSCTK_OPT_EPS_SCALE 2    /* multiplier on the candidate window (default 1); rows whose
-                                     re-scored candidates deviate by more than the scaled
-                                     window's bound are recomputed exactly, so a scale below 1
-                                     costs time, not exactness, as far as the candidates show */
+                                     re-scored candidates, or the keys just above their cut,
+                                     deviate by more than the scaled window's bound are
+                                     recomputed exactly.  The check sees only those keys: a
+                                     scale below 1 costs time, and exactness only as far as a
+                                     pair outside them breaks the bound unseen */
 #define MDSCTK_OPT_REF_SPLITS 3   /* force the number of reference phases (0 = automatic) */
 #define MDSCTK_OPT_SERVE_SLOTS 4  /* batches in flight at once when serving (1..4, default 2) */
 #define MDSCTK_OPT_SERVE_MAX_FRAMES 5 /* query frames per served batch (default 8192) */
@@ -125,7 +136,8 @@ int mdsctk_knn_create(mdsctk_knn_ctx** ctx, int device, int precision);
 /* Release all device memory and the stream. */
 int mdsctk_knn_destroy(mdsctk_knn_ctx* ctx);
 
-/* Text of the last error on this context (never NULL). */
+/* Text of the last failure of a call the calling thread made on this
+ * context (never NULL; empty when there was none). */
 const char* mdsctk_knn_last_error(const mdsctk_knn_ctx* ctx);
 
 int mdsctk_knn_set_option(mdsctk_knn_ctx* ctx, int option, double value);

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "engine.h"
@@ -15,6 +16,10 @@ namespace mdsctk {
 namespace {
 
 thread_local std::string* tl_err_sink = nullptr;
+// Error text of failed calls, per calling thread and context: calls on one
+// context may come from several threads (the serving calls may overlap a
+// synchronous query), and none of them writes another thread's text.
+thread_local std::unordered_map<const mdsctk_knn_ctx*, std::string> tl_ctx_err;
 
 // Candidate window: |s~ - s| <= kappa (G_q + max G_ref) for the approximate
 // score s~ of each contraction form (s = G_q + G_r - 2 lambda).  kappa is an
@@ -78,6 +83,8 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 void set_error_sink(std::string* sink) { tl_err_sink = sink; }
 
+void set_ctx_error(const mdsctk_knn_ctx* c, const std::string& text) { tl_ctx_err[c] = text; }
+
 int fail(mdsctk_knn_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -87,7 +94,7 @@ int fail(mdsctk_knn_ctx* c, int code, const char* fmt, ...) {
   if (tl_err_sink)
     *tl_err_sink = buf;
   else if (c)
-    c->err = buf;
+    tl_ctx_err[c] = buf;
   return code;
 }
 
@@ -174,8 +181,7 @@ int alloc_frames(mdsctk_knn_ctx* c, DevFrames& f, int64_t n, int natoms, int pre
   f.prec = prec == kExact ? kFP32 : prec;
   f.n_tiles = padded_tiles(n);
   f.cap_tiles = f.n_tiles;
-  const int ga = group_atoms(f.prec);
-  f.n_groups = (natoms + ga - 1) / ga;
+  f.n_groups = operand_groups(f.prec, natoms);
   const size_t rows = (size_t)f.n_tiles * kTile;
   int rc;
   if ((rc = dalloc(c, (void**)&f.cen, rows * 3 * sizeof(double), "centroids"))) return rc;
@@ -394,6 +400,7 @@ void slot_release(mdsctk_knn_ctx* c, Slot& s) {
   dfree(c, s.ws_rows);
   dfree(c, s.ws_sym);
   dfree(c, s.ws_relist);
+  dfree(c, s.ws_sort);
   for (auto& e : s.ev)
     if (e) (void)hipEventDestroy(e);
   if (s.st) (void)hipStreamDestroy(s.st);
@@ -423,8 +430,6 @@ int resolve_shape(mdsctk_knn_ctx* c, int32_t k, uint32_t flags, QueryShape* q) {
   q->k_out = q->full ? (int)n_ref : (int)std::min<int64_t>(k, n_ref - 1);
   if (!q->full && q->rank_off == 0) q->k_out = (int)std::min<int64_t>(k, n_ref);
   q->k1 = q->full ? 0 : q->k_out + q->rank_off;
-  if (!q->full && q->k1 > 2048)
-    return fail(c, MDSCTK_ERR_ARG, "query: k=%d exceeds the supported maximum 2047", k);
   q->fused = !q->full && c->prec != kExact && list_capacity_for(q->k1) > 0;
   return 0;
 }
@@ -644,7 +649,7 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
   if (!rc) rc = ensure(c, &s.ws_small, &s.ws_small_bytes, small, "list headers");
   if (rc == MDSCTK_ERR_NOMEM) {  // the list path instead, without the mirrored lists
     release_mirror_lists(c);
-    c->err.clear();
+    tl_ctx_err.erase(c);
     return 1;
   }
   if (rc) return rc;
@@ -765,14 +770,32 @@ int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const in
   }
   double* srow = (double*)ws;
   int* drows = (int*)((char*)ws + row_bytes * batch);
+  // k+1 beyond the LDS select: compaction and a segmented sort in device
+  // memory (the reference has no bound on k, knn_rms.cpp:131, 242)
+  const bool sorted_path = q.k1 > kSelectLdsMaxK1;
+  size_t sort_bytes = 0;
+  if (sorted_path) {
+    if (cv)
+      return fail(c, MDSCTK_ERR_ARG,
+                  "query (fixed workspace): k+1=%d exceeds %d, which needs a sorting workspace",
+                  q.k1, kSelectLdsMaxK1);
+    sort_bytes = sort_rows_scratch(batch, q.k1);
+    int rc = ensure(c, &s.ws_sort, &s.ws_sort_bytes, sort_bytes, "row sort scratch");
+    if (rc) return rc;
+  }
   std::vector<int> ids(batch);
   for (int b0 = 0; b0 < nrows; b0 += batch) {
     const int nb = std::min(batch, nrows - b0);
     for (int i = 0; i < nb; ++i) ids[i] = h_rows ? h_rows[b0 + i] : b0 + i;
     HIP_TRY(c, hipMemcpyAsync(drows, ids.data(), nb * sizeof(int), hipMemcpyHostToDevice, s.st));
     HIP_TRY(c, launch_exact_rows(c->ref, fit, c->w, drows, nb, q.nofit, srow, nullptr, 0, s.st));
-    HIP_TRY(c, launch_select_rows(srow, n_ref, drows, nb, c->w, q.k1, q.k_out, q.rank_off,
-                                  d_dist, d_idx, s.st));
+    if (sorted_path)
+      HIP_TRY(c, launch_select_rows_sorted(srow, n_ref, drows, nb, c->w, q.k1, q.k_out,
+                                           q.rank_off, d_dist, d_idx, s.ws_sort, sort_bytes,
+                                           s.st));
+    else
+      HIP_TRY(c, launch_select_rows(srow, n_ref, drows, nb, c->w, q.k1, q.k_out, q.rank_off,
+                                    d_dist, d_idx, s.st));
     HIP_TRY(c, hipStreamSynchronize(s.st));
   }
   return 0;
@@ -1063,13 +1086,15 @@ int mdsctk_knn_destroy(mdsctk_knn_ctx* c) {
   slot_release(c, c->main);
   for (auto& kv : c->allocs) (void)hipFree(kv.first);
   c->allocs.clear();
+  tl_ctx_err.erase(c);
   delete c;
   return 0;
 }
 
 const char* mdsctk_knn_last_error(const mdsctk_knn_ctx* c) {
   if (!c) return "null context";
-  return c->err.c_str();
+  const auto it = tl_ctx_err.find(c);
+  return it == tl_ctx_err.end() ? "" : it->second.c_str();
 }
 
 int mdsctk_knn_set_option(mdsctk_knn_ctx* c, int option, double value) {

@@ -40,6 +40,8 @@ struct Slot {
   size_t ws_sym_bytes = 0;
   void* ws_relist = nullptr;  // row ids and outputs of rows recomputed through the lists
   size_t ws_relist_bytes = 0;
+  void* ws_sort = nullptr;  // exact rows with k+1 beyond the LDS select: compaction + sort
+  size_t ws_sort_bytes = 0;
   // ev[0] start, ev[1] query prepared, ev[2] before the contraction, ev[3]
   // after it, ev[4] after the merge, ev[5] end; ev[6] after a symmetric self
   // query's landmark pass (so ev[2]..ev[6] is that pass).
@@ -107,7 +109,6 @@ struct mdsctk_knn_ctx {
   int device = 0;
   int prec = mdsctk::kFP32;
   int n_cu = 256;
-  std::string err;
   // Device memory accounting; the serving thread allocates too.
   std::mutex mem_mu;
   size_t budget = 0;  // 0 = unlimited
@@ -144,6 +145,8 @@ int fail(mdsctk_knn_ctx* c, int code, const char* fmt, ...);
 int hip_fail(mdsctk_knn_ctx* c, hipError_t e, const char* what);
 // Directs fail() on the calling thread to `sink` (nullptr = the context).
 void set_error_sink(std::string* sink);
+// The calling thread's error text for context c (mdsctk_knn_last_error).
+void set_ctx_error(const mdsctk_knn_ctx* c, const std::string& text);
 
 int dalloc(mdsctk_knn_ctx* c, void** p, size_t bytes, const char* what);
 void dfree(mdsctk_knn_ctx* c, void* p);

@@ -37,6 +37,15 @@ enum Metric : int { kMetricRMSD = 0, kMetricEuclidean = 1, kMetricCorrelation = 
 // Atoms per operand K-group: split forms hold 16 atoms (one 32x32x16 MFMA
 // step) per group, FP32 holds 8 atoms (four 32x32x2 MFMA steps).
 inline int group_atoms(int prec) { return prec == kFP32 ? 8 : 16; }
+// The split forms' K-loop walks K-groups four at a time (its four stage
+// buffers), so their operand holds a multiple of four groups (the padding
+// atoms are zero and contribute nothing).
+constexpr int kSplitGroupAlign = 4;
+inline int operand_groups(int prec, int natoms) {
+  const int ga = group_atoms(prec);
+  const int g = (natoms + ga - 1) / ga;
+  return prec == kFP32 ? g : (g + kSplitGroupAlign - 1) / kSplitGroupAlign * kSplitGroupAlign;
+}
 // 16-byte operand blocks of 64 lanes per (tile, K-group): 3 planes x parts.
 inline int blocks_per_group(int prec) { return prec == kFP32 ? 3 : 6; }
 // K-groups of zero slack after a frame set's operand: the contraction's
@@ -187,6 +196,15 @@ hipError_t launch_exact_rows(const DevFrames& ref, const DevFrames& fit, const D
 hipError_t launch_select_rows(const double* srow, int64_t n_ref, const int* rows, int nrows,
                               const DevWeights& w, int k1, int k_out, int rank_off,
                               double* out_dist, int* out_idx, hipStream_t st);
+// Any k1 (the LDS select takes k1 <= 2048): the rows' k1 smallest (s, index)
+// compacted in index order, then a stable segmented radix sort; `scratch`
+// holds sort_rows_scratch(nrows, k1) bytes.
+size_t sort_rows_scratch(int nrows, int k1);
+hipError_t launch_select_rows_sorted(const double* srow, int64_t n_ref, const int* rows,
+                                     int nrows, const DevWeights& w, int k1, int k_out,
+                                     int rank_off, double* out_dist, int* out_idx, void* scratch,
+                                     size_t scratch_bytes, hipStream_t st);
+constexpr int kSelectLdsMaxK1 = 2048;
 hipError_t launch_identity_rows(int64_t n_fit, int64_t n_ref, int* out_idx, hipStream_t st);
 // Data vectors: mean[e] over the n vectors x[n][vdim] (FP64).
 hipError_t launch_column_mean(const double* x, int64_t n, int vdim, double* mean, hipStream_t st);

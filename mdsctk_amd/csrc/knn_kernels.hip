@@ -36,6 +36,7 @@
 // so every global load and every LDS read of a fragment is one contiguous
 // 1 KB access per wave.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cfloat>
 #include <climits>
@@ -547,6 +548,62 @@ __device__ __forceinline__ void mma_group(f32x16 (&acc)[9], const uint4* __restr
   }
 }
 
+// One plane c of a split-form K-group: acc[c*3+d] += the three products of
+// A_c (hi, lo) and B_d (hi, lo), d = 0..2.
+template <int PREC>
+__device__ __forceinline__ void mma_plane(f32x16 (&acc)[9], int c, const uint4 ah, const uint4 al,
+                                          const uint4 (&bh)[3], const uint4 (&bl)[3]) {
+#pragma unroll
+  for (int prod = 0; prod < 3; ++prod) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const uint4 av = (prod == 2) ? al : ah;
+      const uint4 bv = (prod == 1) ? bl[d] : bh[d];
+      if constexpr (PREC == kBF16)
+        acc[c * 3 + d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+            __builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv), acc[c * 3 + d], 0, 0, 0);
+      else
+        acc[c * 3 + d] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+            __builtin_bit_cast(f16x8, av), __builtin_bit_cast(f16x8, bv), acc[c * 3 + d], 0, 0, 0);
+    }
+  }
+}
+
+// One stage of the register-pipelined split-form K-loop.  On entry the
+// stage's B fragments (bh, bl) and plane 0's A fragment (ah, al) are in
+// registers (loaded during the previous stage); planes 1 and 2 of A are
+// read from the current buffer (ca) one plane ahead of their MFMAs, and
+// when NEXT the following stage's B fragments (into nbh, nbl) and plane-0
+// A fragment (into ah, al) are read from the next buffer (na, nb) while this
+// stage's MFMAs run.  MMA false: a padding stage (no MFMAs, no reads).
+template <int PREC, bool MMA, bool NEXT>
+__device__ __forceinline__ void pipe_stage(f32x16 (&acc)[9], const uint4* __restrict__ ca,
+                                           const uint4* __restrict__ na,
+                                           const uint4* __restrict__ nb, const uint4 (&bh)[3],
+                                           const uint4 (&bl)[3], uint4 (&nbh)[3], uint4 (&nbl)[3],
+                                           uint4& ah, uint4& al, int lane) {
+  if constexpr (!MMA) return;
+  const uint4 a1h = ca[2 * 64 + lane], a1l = ca[3 * 64 + lane];
+  if constexpr (NEXT) {
+    nbh[0] = nb[0 * 64 + lane];
+    nbl[0] = nb[1 * 64 + lane];
+  }
+  mma_plane<PREC>(acc, 0, ah, al, bh, bl);
+  const uint4 a2h = ca[4 * 64 + lane], a2l = ca[5 * 64 + lane];
+  if constexpr (NEXT) {
+    nbh[1] = nb[2 * 64 + lane];
+    nbl[1] = nb[3 * 64 + lane];
+  }
+  mma_plane<PREC>(acc, 1, a1h, a1l, bh, bl);
+  if constexpr (NEXT) {
+    nbh[2] = nb[4 * 64 + lane];
+    nbl[2] = nb[5 * 64 + lane];
+    ah = na[0 * 64 + lane];
+    al = na[1 * 64 + lane];
+  }
+  mma_plane<PREC>(acc, 2, a2h, a2l, bh, bl);
+}
+
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 // Wave `slot` copies its tile's K-group g (BPG blocks of 64 x 16 B) from HBM
@@ -707,6 +764,20 @@ __device__ __forceinline__ void mirror_append(uint2* __restrict__ ment, int* __r
   }
 }
 
+// The pipelined K-loop's stage end: this wave's copies older than the N
+// most recent have landed, then the workgroup barrier.  LDS reads stay in
+// flight (they read the next stage's buffer, which no copy touches before
+// the following barrier; each is waited for before its MFMAs).
+template <int N>
+__device__ __forceinline__ void stage_barrier_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("" ::: "memory");
+  // vmcnt = N (bits 3:0 and 15:14), expcnt = 7, lgkmcnt = 15 (no wait)
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // FT = fit tiles per workgroup (2 or 4); the workgroup has 2*FT waves, one
 // per (reference tile, fit tile) pair.  MODE: kOutLists, kOutDebug or
 // kOutSym.
@@ -715,13 +786,19 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
   constexpr int BPG = (PREC == kFP32) ? 3 : 6;
   constexpr int TILE_VEC = BPG * 64;
   constexpr int STAGE_VEC = (kRefTilesPerWG + FT) * TILE_VEC;
-  // Three stage buffers (two K-groups in flight while one is consumed),
-  // declared as distinct LDS objects so that the compiler does not serialise
-  // reads of one behind copies into another.  Stage layout:
+  // PIPE (the split forms' full product): a register-pipelined K-loop over
+  // four stage buffers -- stage g computes from fragments read during stage
+  // g-1, reads stage g+1's, and copies stage g+3.  Otherwise three buffers
+  // (two K-groups in flight while one is consumed), each stage read just
+  // before its MFMAs.
+  constexpr bool PIPE = PREC != kFP32 && !TRACE;
+  // Stage buffers, declared as distinct LDS objects so that the compiler does
+  // not serialise reads of one behind copies into another.  Stage layout:
   // [ref tile 0][ref tile 1][fit tile 0] ... [fit tile FT-1].
   __shared__ uint4 sbuf0[STAGE_VEC];
   __shared__ uint4 sbuf1[STAGE_VEC];
   __shared__ uint4 sbuf2[STAGE_VEC];
+  __shared__ uint4 sbuf3[PIPE ? STAGE_VEC : 1];
   __shared__ double sGr[2][2 * kTile];  // G~ of the current / next reference tile pair
   // kOutSym: shared thresholds of the current / next reference tile pair's
   // frames (the filter of their mirrored scores)
@@ -735,9 +812,10 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
   const int ri = wave & 1;   // reference tile of this wave
   const int fi = wave >> 1;  // fit tile of this wave
   const int ng = P.n_groups;
-  // Stages per tile, a multiple of three so that the buffer rotation lines
-  // up with tile boundaries; stages g >= ng copy padding and skip the MFMAs.
-  const int ng3 = (ng + 2) / 3 * 3;
+  // Stages per tile, a multiple of the buffer count so that the buffer
+  // rotation lines up with tile boundaries; stages g >= ng copy padding and
+  // skip the MFMAs.
+  const int ngs = PIPE ? ng : (ng + 2) / 3 * 3;
 
   // Waves 0..kSlots-1 each copy one stage slot (slot = wave); the others
   // only compute.
@@ -752,6 +830,9 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
   const uint4* const b0 = sbuf0 + (kRefTilesPerWG + fi) * TILE_VEC;
   const uint4* const b1 = sbuf1 + (kRefTilesPerWG + fi) * TILE_VEC;
   const uint4* const b2 = sbuf2 + (kRefTilesPerWG + fi) * TILE_VEC;
+  uint4* const d3 = sbuf3 + (PIPE && copier ? wave : 0) * TILE_VEC;
+  const uint4* const a3 = sbuf3 + (PIPE ? ri : 0) * TILE_VEC;
+  const uint4* const b3 = sbuf3 + (PIPE ? kRefTilesPerWG + fi : 0) * TILE_VEC;
   const int fit_blocks = P.fit_tiles / FT;
 
   // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
@@ -791,7 +872,7 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
     int cnt = 0;
     float thr = INFINITY, ovf = INFINITY;
 
-    // Stage s of this item is K-group g = s % ng3 of tile t = t0 + s / ng3.  Wave
+    // Stage s of this item is K-group g = s % ngs of tile t = t0 + s / ngs.  Wave
     // w < 2 copies ref tile 2p+w (p = ph + t*R), wave 2..FT+1 fit tile
     // FT*fb+w-2.  Stages past the item's last tile re-read its last tile
     // (never used).
@@ -805,7 +886,7 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
     };
     auto dma_stage = [&](int t, int g, uint4* dst) {
       if (!copier) return;
-      if (g >= ng3) { g -= ng3; ++t; }
+      if (g >= ngs) { g -= ngs; ++t; }
       const uint4* src = src_of(t, g) + lane;
 #pragma unroll
       for (int j = 0; j < BPG; ++j)
@@ -821,7 +902,19 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
     }
     dma_stage(t0, 0, d0);
     dma_stage(t0, 1, d1);
-    stage_barrier<BPG>();
+    if constexpr (PIPE) dma_stage(t0, 2, d2);
+    stage_barrier<BPG>();  // stage 0 (and with PIPE stage 1) landed
+    // PIPE: the fragments of the stage about to be computed (B double-buffered)
+    uint4 bh0[3], bl0[3], bh1[3], bl1[3], ah, al;
+    if constexpr (PIPE) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        bh0[d] = b0[(2 * d) * 64 + lane];
+        bl0[d] = b0[(2 * d + 1) * 64 + lane];
+      }
+      ah = a0[0 * 64 + lane];
+      al = a0[1 * 64 + lane];
+    }
 
     for (int t = t0; t < n_tiles; ++t) {
       const int p = ph + t * P.phases;
@@ -841,22 +934,48 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
       f32x16 acc[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) acc[q] = (f32x16){0};
-      // Stage g lives in buffer g % 3; while it is consumed, stage g+2 (in
-      // this tile or the next) is being copied.
-      // The copy of stage g+2 is issued after stage g's MFMAs: the MFMAs
-      // start right after the barrier, and the copy (into the buffer every
-      // wave finished reading before that barrier) overlaps their tail.
-      // Measured 6 % faster than issuing the copy first.
-      for (int g = 0; g < ng3; g += 3) {
-        mma_group<PREC, TRACE>(acc, a0, b0, lane);
-        dma_stage(t, g + 2, d2);
-        stage_barrier<BPG>();
-        if (g + 1 < ng) mma_group<PREC, TRACE>(acc, a1, b1, lane);
-        dma_stage(t, g + 3, d0);
-        stage_barrier<BPG>();
-        if (g + 2 < ng) mma_group<PREC, TRACE>(acc, a2, b2, lane);
-        dma_stage(t, g + 4, d1);
-        stage_barrier<BPG>();
+      if constexpr (PIPE) {
+        // Stage g lives in buffer g % 4 (ng is a multiple of 4 here: the
+        // operand is padded with zero atoms).  At the start of stage g its
+        // fragments are in registers (read during stage g-1) and stage g+1
+        // has landed; stage g reads stage g+1's fragments while its MFMAs
+        // run, and copies stage g+3 into the buffer of stage g-1, which every
+        // wave finished reading before the previous barrier.  The tile's last
+        // stage reads the next tile's first one, so the pipeline runs through
+        // the epilogue (past the item's last tile it reads a re-copied tile,
+        // never used).
+        for (int g = 0; g < ng; g += 4) {
+          pipe_stage<PREC, true, true>(acc, a0, a1, b1, bh0, bl0, bh1, bl1, ah, al, lane);
+          dma_stage(t, g + 3, d3);
+          stage_barrier_vm<BPG>();
+          pipe_stage<PREC, true, true>(acc, a1, a2, b2, bh1, bl1, bh0, bl0, ah, al, lane);
+          dma_stage(t, g + 4, d0);
+          stage_barrier_vm<BPG>();
+          pipe_stage<PREC, true, true>(acc, a2, a3, b3, bh0, bl0, bh1, bl1, ah, al, lane);
+          dma_stage(t, g + 5, d1);
+          stage_barrier_vm<BPG>();
+          pipe_stage<PREC, true, true>(acc, a3, a0, b0, bh1, bl1, bh0, bl0, ah, al, lane);
+          dma_stage(t, g + 6, d2);
+          stage_barrier_vm<BPG>();
+        }
+      } else {
+        // Stage g lives in buffer g % 3; while it is consumed, stage g+2 (in
+        // this tile or the next) is being copied.
+        // The copy of stage g+2 is issued after stage g's MFMAs: the MFMAs
+        // start right after the barrier, and the copy (into the buffer every
+        // wave finished reading before that barrier) overlaps their tail.
+        // Measured 6 % faster than issuing the copy first.
+        for (int g = 0; g < ngs; g += 3) {
+          mma_group<PREC, TRACE>(acc, a0, b0, lane);
+          dma_stage(t, g + 2, d2);
+          stage_barrier<BPG>();
+          if (g + 1 < ng) mma_group<PREC, TRACE>(acc, a1, b1, lane);
+          dma_stage(t, g + 3, d0);
+          stage_barrier<BPG>();
+          if (g + 2 < ng) mma_group<PREC, TRACE>(acc, a2, b2, lane);
+          dma_stage(t, g + 4, d1);
+          stage_barrier<BPG>();
+        }
       }
       // Publish the next tile's G~ and thresholds (read by its epilogue,
       // after at least one more barrier).
@@ -1091,6 +1210,9 @@ __device__ void block_bitonic_sort(double* s, int* idx, int n_pow2) {
   __syncthreads();
 }
 
+// Keys just above a row's cut that the merge re-scores to check the window.
+constexpr int kMergeGuard = 32;
+
 // STAGE: frames whose query frame fits kMergeStageAtoms are re-scored
 // against an LDS copy of it, so each candidate reads only its reference
 // frame from memory.
@@ -1110,6 +1232,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
   __shared__ double cand_s[CAND];
   __shared__ float cand_key[CAND];
   __shared__ int s_bad;
+  // the guard: keys just above the cut, re-scored to check the window
+  __shared__ int guard_idx[kMergeGuard];
+  __shared__ float guard_key[kMergeGuard];
+  __shared__ double guard_s[kMergeGuard];
+  __shared__ int s_ng;
 
   const int* cnt = P.cnt + q * P.nl;
   const uint2* ent = P.ent + q * P.nl * (int64_t)P.cap;
@@ -1126,7 +1253,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
     for (int i = tid; i < mc; i += 256) f(ment[i]);
   };
 
-  if (tid == 0) { s_total = mc; s_nc = 0; s_flag = 0; s_bad = 0; }
+  if (tid == 0) { s_total = mc; s_nc = 0; s_flag = 0; s_bad = 0; s_ng = 0; }
   __syncthreads();
   {
     int part = 0;
@@ -1166,20 +1293,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
   }
   const float tau = __uint_as_float(prefix);
   const float cut = tau * (1.0f + 2.4e-7f) + P.eps2[q];
+  // The guard range: one more window width above the cut.  A window too
+  // narrow for its row would drop pairs whose keys lie just above the cut,
+  // so up to kMergeGuard of those are re-scored as well.
+  const float guard_cut = cut + P.eps2[q];
   for (int l = tid; l < P.nl; l += 256)
     if (ovf[l] <= cut) s_flag = 1;
   if (tid == 0 && P.ment && P.movf[q] <= cut) s_flag = 1;
   visit([&](const uint2 e) {
-    if (__uint_as_float(e.x) <= cut) {
+    const float key = __uint_as_float(e.x);
+    if (key <= cut) {
       const int pos = atomicAdd(&s_nc, 1);
       if (pos < CAND) {
         cand_idx[pos] = (int)e.y;
-        cand_key[pos] = __uint_as_float(e.x);
+        cand_key[pos] = key;
+      }
+    } else if (key <= guard_cut) {
+      const int pos = atomicAdd(&s_ng, 1);
+      if (pos < kMergeGuard) {
+        guard_idx[pos] = (int)e.y;
+        guard_key[pos] = key;
       }
     }
   });
   __syncthreads();
   const int nc = s_nc;
+  const int ng = min(s_ng, kMergeGuard);
   if (s_flag || nc > CAND) {
     if (tid == 0) P.fallback[q] = 1;
     return;
@@ -1194,19 +1333,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
     for (int i = tid; i < P.src.natoms; i += 256) q_stage[i] = fx[i];
     __syncthreads();
   }
-  for (int c = wave; c < nc; c += 4) {
-    const double s = wave_pair_score<STAGE>(P.src, cand_idx[c], q, P.nofit);
-    if (lane == 0) cand_s[c] = s;
+  for (int c = wave; c < nc + ng; c += 4) {
+    const int r = c < nc ? cand_idx[c] : guard_idx[c - nc];
+    const double s = wave_pair_score<STAGE>(P.src, r, q, P.nofit);
+    if (lane == 0) {
+      if (c < nc)
+        cand_s[c] = s;
+      else
+        guard_s[c - nc] = s;
+    }
   }
   __syncthreads();
   // The window assumes that every approximate score lies within
-  // eps = eps2 / 2 of the exact one.  The candidates' exact scores are known
-  // now: a row where one deviates further takes the exact path.  The keys
-  // are rounded to FP32, hence the relative allowance.
+  // eps = eps2 / 2 of the exact one.  The exact scores of the candidates and
+  // of the guard keys are known now: a row where one deviates further takes
+  // the exact path.  The keys are rounded to FP32, hence the relative
+  // allowance.
   {
     const double eps = 0.5 * (double)P.eps2[q];
-    for (int c = tid; c < nc; c += 256) {
-      const double s = cand_s[c], kq = (double)cand_key[c];
+    for (int c = tid; c < nc + ng; c += 256) {
+      const double s = c < nc ? cand_s[c] : guard_s[c - nc];
+      const double kq = (double)(c < nc ? cand_key[c] : guard_key[c - nc]);
       if (fabs(kq - s) > eps + 1.2e-7 * fmax(kq, s)) s_bad = 1;
     }
   }
@@ -1465,26 +1612,18 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(ExactParams P) {
   }
 }
 
-// Top-k1 by (s, index) over one exact score row per workgroup.
-__global__ __launch_bounds__(256) void select_rows_kernel(const double* __restrict__ srow,
-                                                          int64_t n_ref,
-                                                          const int* __restrict__ rows,
-                                                          double W, double out_scale,
-                                                          int k1, int k_out,
-                                                          int rank_off, double* out_dist,
-                                                          int* out_idx) {
-  const int64_t ri = blockIdx.x;
-  const int64_t q = rows ? rows[ri] : ri;
-  const double* s = srow + ri * n_ref;
+// The k1-th smallest (s, index) of one exact score row, by radix select on
+// the 64-bit keys (s >= 0, so the bit patterns order like the values) and,
+// among keys equal to it, on the index: every entry with key < prefix, or
+// key == prefix and index <= ip, is among the k1 smallest (exactly k1 of
+// them).  256 threads.
+__device__ void row_threshold(const double* __restrict__ s, int64_t n_ref, int k1,
+                              unsigned long long* prefix_out, unsigned* ip_out) {
   const int tid = threadIdx.x;
   __shared__ unsigned hist[256];
   __shared__ unsigned long long s_prefix;
-  __shared__ int s_kk, s_nc;
+  __shared__ int s_kk;
   __shared__ unsigned s_iprefix;
-  __shared__ int cand_idx[kCandMax];
-  __shared__ double cand_s[kCandMax];
-
-  // rank k1-1 among the 64-bit keys (s >= 0, so the bit pattern is monotone)
   unsigned long long prefix = 0;
   int kk = k1 - 1;
   for (int shift = 56; shift >= 0; shift -= 8) {
@@ -1538,11 +1677,38 @@ __global__ __launch_bounds__(256) void select_rows_kernel(const double* __restri
     ip = s_iprefix;
     ik = s_kk;
   }
+  *prefix_out = prefix;
+  *ip_out = ip;
+}
+
+__device__ __forceinline__ bool among_smallest(double v, int64_t i, unsigned long long prefix,
+                                               unsigned ip) {
+  const unsigned long long k = (unsigned long long)__double_as_longlong(v);
+  return k < prefix || (k == prefix && (unsigned)i <= ip);
+}
+
+// Top-k1 by (s, index) over one exact score row per workgroup (k1 <= kCandMax).
+__global__ __launch_bounds__(256) void select_rows_kernel(const double* __restrict__ srow,
+                                                          int64_t n_ref,
+                                                          const int* __restrict__ rows,
+                                                          double W, double out_scale,
+                                                          int k1, int k_out,
+                                                          int rank_off, double* out_dist,
+                                                          int* out_idx) {
+  const int64_t ri = blockIdx.x;
+  const int64_t q = rows ? rows[ri] : ri;
+  const double* s = srow + ri * n_ref;
+  const int tid = threadIdx.x;
+  __shared__ int s_nc;
+  __shared__ int cand_idx[kCandMax];
+  __shared__ double cand_s[kCandMax];
+  unsigned long long prefix;
+  unsigned ip;
+  row_threshold(s, n_ref, k1, &prefix, &ip);
   if (tid == 0) s_nc = 0;
   __syncthreads();
   for (int64_t i = tid; i < n_ref; i += 256) {
-    const unsigned long long k = (unsigned long long)__double_as_longlong(s[i]);
-    if (k < prefix || (k == prefix && (unsigned)i <= ip)) {
+    if (among_smallest(s[i], i, prefix, ip)) {
       const int pos = atomicAdd(&s_nc, 1);
       if (pos < kCandMax) {
         cand_idx[pos] = (int)i;
@@ -1564,6 +1730,65 @@ __global__ __launch_bounds__(256) void select_rows_kernel(const double* __restri
     out_dist[q * k_out + j] = out_scale * sqrt(fmax(cand_s[src], 0.0) / W);
     out_idx[q * k_out + j] = cand_idx[src];
   }
+}
+
+// Any k1 (beyond the LDS candidate arrays): the k1 smallest (s, index) of
+// each row, in index order, to [ri][k1] key / index arrays (the keys are the
+// scores' bit patterns).  Thread t takes a contiguous run of the row, so a
+// block-wide scan of the per-thread counts gives every entry its slot.
+__global__ __launch_bounds__(256) void compact_rows_kernel(const double* __restrict__ srow,
+                                                           int64_t n_ref, int k1,
+                                                           unsigned long long* __restrict__ keys,
+                                                           int* __restrict__ idx) {
+  const int64_t ri = blockIdx.x;
+  const double* s = srow + ri * n_ref;
+  const int tid = threadIdx.x;
+  __shared__ int part[256];
+  unsigned long long prefix;
+  unsigned ip;
+  row_threshold(s, n_ref, k1, &prefix, &ip);
+  const int64_t per = (n_ref + 255) / 256;
+  const int64_t lo = min(n_ref, tid * per), hi = min(n_ref, lo + per);
+  int cnt = 0;
+  for (int64_t i = lo; i < hi; ++i) cnt += among_smallest(s[i], i, prefix, ip) ? 1 : 0;
+  part[tid] = cnt;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int pos = part[tid] - cnt;
+  unsigned long long* kr = keys + ri * (int64_t)k1;
+  int* ir = idx + ri * (int64_t)k1;
+  for (int64_t i = lo; i < hi && pos < k1; ++i)
+    if (among_smallest(s[i], i, prefix, ip)) {
+      kr[pos] = (unsigned long long)__double_as_longlong(s[i]);
+      ir[pos] = (int)i;
+      ++pos;
+    }
+}
+
+// Sorted [ri][k1] rows to the outputs: ranks rank_off .. rank_off+k_out-1.
+__global__ void write_sorted_rows_kernel(const unsigned long long* __restrict__ keys,
+                                         const int* __restrict__ idx, const int* __restrict__ rows,
+                                         int k1, int k_out, int rank_off, double W,
+                                         double out_scale, double* __restrict__ out_dist,
+                                         int* __restrict__ out_idx) {
+  const int64_t ri = blockIdx.x;
+  const int64_t q = rows ? rows[ri] : ri;
+  for (int j = threadIdx.x; j < k_out; j += blockDim.x) {
+    const int64_t src = ri * (int64_t)k1 + j + rank_off;
+    const double sv = __longlong_as_double((long long)keys[src]);
+    out_dist[q * k_out + j] = out_scale * sqrt(fmax(sv, 0.0) / W);
+    out_idx[q * k_out + j] = idx[src];
+  }
+}
+
+__global__ void segment_offsets_kernel(int* off, int n, int k1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) off[i] = i * k1;
 }
 
 // thr may be null: the shared thresholds are kept (set by a landmark pass).
@@ -1788,6 +2013,8 @@ hipError_t launch_contract(const DevFrames& ref, const DevFrames& fit, const Lis
   P.movf = mirror ? mirror->ovf : nullptr;
   P.mcap = mirror ? mirror->cap : 0;
   if (P.ph_run > ref_phases || ref_phases < 1) return hipErrorInvalidValue;
+  // the split forms' pipelined K-loop walks K-groups four at a time
+  if (ref.prec != kFP32 && (ref.n_groups % kSplitGroupAlign)) return hipErrorInvalidValue;
   if (mirror && (fit.n != ref.n || fit.n_tiles != ref.n_tiles || !L.thr || !mirror->ent ||
                  !mirror->cnt || !mirror->ovf || mirror->cap < 1 || P.ph_run != ref_phases))
     return hipErrorInvalidValue;
@@ -1985,6 +2212,51 @@ hipError_t launch_select_rows(const double* srow, int64_t n_ref, const int* rows
   if (k1 > kCandMax) return hipErrorInvalidValue;
   hipLaunchKernelGGL(select_rows_kernel, dim3((unsigned)nrows), dim3(256), 0, st, srow, n_ref,
                      rows, w.W, w.out_scale, k1, k_out, rank_off, out_dist, out_idx);
+  return hipGetLastError();
+}
+
+size_t sort_rows_scratch(int nrows, int k1) {
+  size_t temp = 0;
+  (void)hipcub::DeviceSegmentedRadixSort::SortPairs<unsigned long long, int, const int*>(
+      nullptr, temp, nullptr, nullptr, nullptr, nullptr, nrows * k1, nrows, nullptr, nullptr);
+  const size_t items = (size_t)nrows * k1;
+  return 2 * items * (sizeof(unsigned long long) + sizeof(int)) + (size_t)(nrows + 1) * 4 + temp +
+         1024;
+}
+
+hipError_t launch_select_rows_sorted(const double* srow, int64_t n_ref, const int* rows,
+                                     int nrows, const DevWeights& w, int k1, int k_out,
+                                     int rank_off, double* out_dist, int* out_idx, void* scratch,
+                                     size_t scratch_bytes, hipStream_t st) {
+  if (nrows <= 0) return hipSuccess;
+  if ((int64_t)nrows * k1 > INT32_MAX || k1 > n_ref) return hipErrorInvalidValue;
+  const size_t items = (size_t)nrows * k1;
+  char* p = (char*)scratch;
+  auto take = [&](size_t b) {
+    char* r = p;
+    p += (b + 255) & ~size_t(255);
+    return (void*)r;
+  };
+  auto* k_in = (unsigned long long*)take(items * sizeof(unsigned long long));
+  auto* k_out_b = (unsigned long long*)take(items * sizeof(unsigned long long));
+  auto* i_in = (int*)take(items * sizeof(int));
+  auto* i_out = (int*)take(items * sizeof(int));
+  auto* off = (int*)take((size_t)(nrows + 1) * sizeof(int));
+  const size_t used = (size_t)(p - (char*)scratch);
+  if (used > scratch_bytes) return hipErrorInvalidValue;
+  size_t temp = scratch_bytes - used;
+  hipLaunchKernelGGL(compact_rows_kernel, dim3((unsigned)nrows), dim3(256), 0, st, srow, n_ref, k1,
+                     k_in, i_in);
+  hipLaunchKernelGGL(segment_offsets_kernel, dim3((unsigned)((nrows + 256) / 256)), dim3(256), 0,
+                     st, off, nrows, k1);
+  // stable radix sort on the score bits: rows come in index order, so ties
+  // keep ascending indices
+  hipError_t e = hipcub::DeviceSegmentedRadixSort::SortPairs(
+      (void*)p, temp, k_in, k_out_b, i_in, i_out, (int)items, nrows, (const int*)off,
+      (const int*)off + 1, 0, 64, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(write_sorted_rows_kernel, dim3((unsigned)nrows), dim3(256), 0, st, k_out_b,
+                     i_out, rows, k1, k_out, rank_off, w.W, w.out_scale, out_dist, out_idx);
   return hipGetLastError();
 }
 

@@ -627,10 +627,7 @@ int server_poll(mdsctk_knn_ctx* c, int64_t id, mdsctk_knn_request_info* info, bo
   if (!s) return fail(c, MDSCTK_ERR_STATE, "poll: no request was submitted");
   std::string err;
   const int rc = s->poll(id, info, wait, &err);
-  if (rc < 0 && rc != MDSCTK_ERR_PENDING) {
-    std::lock_guard<std::mutex> g(c->server_mu);
-    c->err = err;
-  }
+  if (rc < 0 && rc != MDSCTK_ERR_PENDING) set_ctx_error(c, err);  // the polling thread's text
   return rc;
 }
 

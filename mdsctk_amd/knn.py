@@ -221,6 +221,10 @@ class KnnContext:
         self.n_ref = 0
         self.natoms = 0
 
+    def last_error(self) -> str:
+        """The calling thread's last failure text on this context."""
+        return self._L.mdsctk_knn_last_error(self._h).decode()
+
     def close(self):
         if getattr(self, "_h", None):
             self._L.mdsctk_knn_destroy(self._h)

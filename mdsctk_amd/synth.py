@@ -100,7 +100,23 @@ def masses(natoms: int, kind: str = "uniform", seed: int = 0) -> np.ndarray:
         return np.resize(pat, natoms).astype(np.float32)
     if kind == "random":
         return np.random.default_rng(seed).uniform(1.0, 32.0, natoms).astype(np.float32)
+    if kind == "wide":
+        # hydrogen to heavy-metal range
+        return np.random.default_rng(seed).uniform(1.0, 200.0, natoms).astype(np.float32)
     raise ValueError(kind)
+
+
+def extended_chain(n_frames: int, natoms: int, seed: int, sigma: float = 0.05) -> np.ndarray:
+    """Nearly straight chains: atoms 0.38 nm apart along a line with
+    N(0, sigma^2) transverse and axial noise per frame, then a random rigid
+    motion.  Close to collinear, so two of the key matrix's eigenvalues
+    nearly coincide (the hard case of the QCP Newton iteration)."""
+    rng = np.random.default_rng([seed, 3])
+    line = np.zeros((natoms, 3))
+    line[:, 0] = 0.38 * np.arange(natoms)
+    line -= line.mean(axis=0)
+    pre = line[None] + sigma * rng.normal(size=(n_frames, natoms, 3))
+    return _rigid(pre, rng)
 
 
 def data_vectors(n: int, vdim: int, seed: int, kind: str = "clusters",

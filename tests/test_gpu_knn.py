@@ -68,6 +68,49 @@ def test_approx_scores_within_window(prec, gen, natoms, nofit):
     assert rel.max() < kappa and rel_self.max() < kappa
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("gen", ["metastable-2000-wide", "extended-2000", "extended-1000-wide",
+                                 "metastable-1000-dups"])
+def test_window_at_large_frames(prec, gen):
+    """The candidate window at the large-frame shapes: 2000-atom frames
+    (the mixed serving workload's largest set) with masses spanning 1-200,
+    nearly collinear extended chains (near-degenerate rotations), and the
+    1000-atom metastable frames with 5 % near-duplicates at 1e-4 nm.  Every
+    approximate score, cross and self, stays inside kappa (G_q + max G_ref);
+    and queries at a window 10^4 times too narrow are caught by the run-time
+    check (rows recomputed, outputs unchanged)."""
+    kind, natoms, extra = gen.split("-")
+    natoms = int(natoms)
+    if kind == "metastable":
+        allf = synth.metastable_trajectory(176, natoms, natoms + 3, n_centres=4,
+                                           dup_fraction=0.05 if extra == "dups" else 0.2)
+    else:
+        allf = synth.extended_chain(176, natoms, natoms)
+    w = synth.masses(natoms, "wide", 5) if extra == "wide" else synth.masses(natoms, "uniform")
+    ref, fit = allf[:96], allf[96:]
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(ref, w)
+        a, e = ctx.debug_scores(fit)
+        a_self, e_self = ctx.debug_scores(None)
+        d0, i0 = ctx.query(fit, 5)
+        kappa = ctx.stats()["eps_rel"]
+        ctx.set_option(knn.OPT_EPS_SCALE, 1e-4)
+        d1, i1 = ctx.query(fit, 5)
+        fb = ctx.stats()["n_fallback_rows"]
+    G = np.array([oracle.center(x, w)[0] for x in ref])
+    Gf = np.array([oracle.center(x, w)[0] for x in fit])
+    rel = np.abs(a - e) / (Gf[:, None] + G.max())
+    rel_self = np.abs(a_self - e_self) / (G[:, None] + G.max())
+    print(prec, gen, "max rel", rel.max(), "self", rel_self.max(), "kappa", kappa,
+          "narrow-window fallback rows", fb)
+    assert rel.max() < kappa and rel_self.max() < kappa
+    assert fb > 0
+    np.testing.assert_array_equal(d1, d0)
+    np.testing.assert_array_equal(i1, i0)
+    d_o, i_o = oracle.knn_exact(ref, fit[:8], w, 5)
+    check_knn(ref, fit[:8], w, d0[:8], i0[:8], d_o, i_o)
+
+
 @pytest.mark.parametrize("prec", PRECS)
 def test_knn_small_vs_oracle(prec):
     ref, fit, w = _case(300, 170, 37, 3, "random")
@@ -154,3 +197,30 @@ def test_c2_workload_full_size(prec):
     d_o, i_o = oracle.knn_exact(ref, fit[rows], w, k)
     check_knn(ref, fit[rows], w, d[rows], i[rows], d_o, i_o)
     print(prec, "C2", st["ms_total"], "ms", st["n_candidates"] / 50_000, "candidates per query")
+
+
+@pytest.mark.parametrize("prec", ["bf16", "exact"])
+def test_k_beyond_2048(prec):
+    """k has no upper bound (knn_rms.cpp:131, 224-225, 242): k+1 above the
+    LDS select's 2048 takes the compaction + segmented sort path; rows in
+    (distance, index) order, equal to the oracle's, for cross and self
+    queries, with rank 0 dropped or kept, and k clamped to n_ref - 1."""
+    ref, fit, w = _case(3000, 40, 24, 17, "backbone")
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(ref, w)
+        d, i = ctx.query(fit, 2500)
+        d_s, i_s = ctx.query(None, 2047)           # k+1 = 2048: the LDS select
+        d_t, i_t = ctx.query(None, 2048)           # k+1 = 2049: the sorted path
+        d_k, i_k = ctx.query(fit[:5], 5000, knn.KNN_KEEP_RANK0)  # clamped to 3000
+    assert d.shape == (40, 2500) and d_k.shape == (5, 3000)
+    d_o, i_o = oracle.knn_exact(ref, fit, w, 2500)
+    check_knn(ref, fit, w, d, i, d_o, i_o)
+    np.testing.assert_array_equal(d_t[:, :2047], d_s)
+    np.testing.assert_array_equal(i_t[:, :2047], i_s)
+    rows = [0, 1500, 2999]
+    d_o, i_o = oracle.knn_exact(ref, ref[rows], w, 2048)
+    check_knn(ref, ref[rows], w, d_t[rows], i_t[rows], d_o, i_o)
+    d_o, i_o = oracle.knn_exact(ref, fit[:5], w, 3000, keep_rank0=True)
+    m = min(d_o.shape[1], d_k.shape[1])
+    check_knn(ref, fit[:5], w, d_k[:, :m], i_k[:, :m], d_o[:, :m], i_o[:, :m])
+    assert (np.sort(i_k, axis=1) == np.arange(3000)[None, :]).all()

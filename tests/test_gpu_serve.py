@@ -163,3 +163,60 @@ def test_serve_errors_and_retirement():
         r2 = ctx.wait(ctx.submit(q, 3))
     d_o, i_o = oracle.knn_exact(ref2, q, w, 3)
     check_knn(ref2, q, w, r2.dist, r2.idx, d_o, i_o)
+
+
+def test_submit_poll_wait_from_several_threads():
+    """Four host threads submit, poll and wait at the same time (the C ABI
+    allows the serving calls from any thread), while the main thread runs a
+    synchronous query on the same context; every result equals the same
+    request run synchronously, and errors stay on the thread that caused
+    them."""
+    import threading
+    ref, w = _set(1500, 40, 33)
+    qs = _queries([1, 7, 33, 128, 257, 64, 3, 900], 40, 33)
+    k = 6
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_reference(ref, w)
+        want = [ctx.query(q, k) for q in qs]
+        got = {}
+        errors = []
+
+        def worker(t):
+            try:
+                mine = []
+                for rep in range(6):
+                    for j in range(t, len(qs), 4):
+                        mine.append((j, ctx.submit(qs[j], k)))
+                for j, r in mine:
+                    if j % 2:
+                        while not ctx.poll(r):
+                            pass
+                    else:
+                        ctx.wait(r)
+                    got.setdefault(j, []).append((r.dist.copy(), r.idx.copy()))
+                # a failing call on this thread: its error text is this thread's
+                import ctypes
+                q0 = np.ascontiguousarray(qs[0], np.float32)
+                out_d = np.zeros((1, k)), np.zeros((1, k), np.int32)
+                rid = ctypes.c_int64()
+                rc = ctx._L.mdsctk_knn_submit(ctx._h, q0.ctypes.data, 1, -1, 0,
+                                              out_d[0].ctypes.data, out_d[1].ctypes.data,
+                                              ctypes.byref(rid))
+                assert rc == knn.ERR_ARG and "k must be" in ctx.last_error()
+            except Exception as e:  # pragma: no cover - reported below
+                errors.append(e)
+
+        threads = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+        for th in threads:
+            th.start()
+        d_sync, i_sync = ctx.query(qs[4], k)  # a synchronous query meanwhile
+        for th in threads:
+            th.join()
+        assert not errors, errors
+        assert ctx.last_error() == ""  # the workers' failures stayed on their threads
+    np.testing.assert_array_equal(d_sync, want[4][0])
+    for j, runs in got.items():
+        assert len(runs) == 6
+        for d, i in runs:
+            np.testing.assert_array_equal(d, want[j][0])
+            np.testing.assert_array_equal(i, want[j][1])

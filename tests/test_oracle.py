@@ -179,3 +179,40 @@ def test_small_isotropic_perturbation():
     nofit = oracle.rmsd_nm(a, b, w, fit=False)
     assert fit <= nofit + 1e-12
     assert fit >= 0.97 * nofit
+
+
+def test_cross_covariance_and_qcp_compose_to_the_rmsd():
+    """The pieces the device epilogue and re-score compute, one at a time:
+    the FP64 cross-covariance S = sum w (a - ca)(b - cb)^T against numpy's
+    weighted product of centred frames; lambda_max of Horn's key matrix from S
+    (QCP) against numpy's eigen-solve of the same 4x4 matrix; and
+    s = G_a + G_b - 2 lambda turned into an RMSD against the oracle's
+    rmsd_nm and a numpy SVD-Kabsch fit with a proper rotation."""
+    rng = np.random.default_rng(12)
+    for natoms in (3, 17, 304):
+        w = rng.uniform(1.0, 32.0, natoms).astype(np.float32)
+        a = rng.normal(size=(natoms, 3)).astype(np.float32) * 2.0
+        b = (a + 0.3 * rng.normal(size=(natoms, 3))).astype(np.float32)
+        S, ga, gb = oracle.cross_cov(a, b, w)
+        wd = w.astype(np.float64)
+        ac = a - (wd[:, None] * a).sum(0) / wd.sum()
+        bc = b - (wd[:, None] * b).sum(0) / wd.sum()
+        S_np = (wd[:, None] * ac).T @ bc
+        assert np.abs(S - S_np).max() <= 1e-9 * np.abs(S_np).max()
+        assert abs(ga - (wd[:, None] * ac * ac).sum()) <= 1e-9 * ga
+        assert abs(gb - (wd[:, None] * bc * bc).sum()) <= 1e-9 * gb
+        xx, xy, xz, yx, yy, yz, zx, zy, zz = S_np.reshape(-1)
+        K = np.array([[xx + yy + zz, yz - zy, zx - xz, xy - yx],
+                      [yz - zy, xx - yy - zz, xy + yx, zx + xz],
+                      [zx - xz, xy + yx, -xx + yy - zz, yz + zy],
+                      [xy - yx, zx + xz, yz + zy, -xx - yy + zz]])
+        lam, _ = oracle.qcp_lambda(S, ga, gb)
+        lam_np = np.linalg.eigvalsh(K)[-1]
+        assert abs(lam - lam_np) <= 1e-9 * abs(lam_np)
+        rmsd = np.sqrt(max(ga + gb - 2.0 * lam, 0.0) / wd.sum())
+        assert abs(rmsd - oracle.rmsd_nm(a, b, w)) <= 1e-9
+        # Kabsch: the proper rotation maximising tr(R S) gives the same minimum
+        U, sv, Vt = np.linalg.svd(S_np)
+        dsign = np.sign(np.linalg.det(U @ Vt))
+        tr = sv[0] + sv[1] + dsign * sv[2]
+        assert abs(lam - tr) <= 1e-9 * abs(tr)
