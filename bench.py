@@ -452,8 +452,8 @@ def main():
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16", "fp32"])
     ap.add_argument("--fp32-steps", type=int, default=1, help="steps of the FP32 comparator (0 = skip)")
     ap.add_argument("--self-symmetric", type=int, default=1,
-                    help="score each unordered pair once through the n x n score matrix (1) or "
-                         "every pair through the candidate lists (0)")
+                    help="score each unordered pair once, the upper block triangle with mirrored "
+                         "lists (1), or every ordered pair through the candidate lists (0)")
     ap.add_argument("--list-steps", type=int, default=1,
                     help="steps of the candidate-list path, reported beside a symmetric headline "
                          "(0 = skip)")

@@ -182,7 +182,7 @@ int mdsctk_knn_query_device(mdsctk_knn_ctx* ctx, const float* d_xyz_nm, int64_t 
  * the query frames' derived arrays (about 12 natoms + 64 bytes each), their
  * candidate lists (the reference phases are lowered until they fit), and
  * one exact FP64 row of 8 n_ref bytes.  A self query (d_xyz_nm NULL) takes
- * the candidate lists, never the score matrix.  Frames only (a context
+ * the candidate lists, never the symmetric form.  Frames only (a context
  * holding data vectors is refused).  Every call on one GPU shares that
  * GPU's region, whatever the context: calls that overlap on one GPU race,
  * so callers serialise them. */

@@ -79,7 +79,7 @@ def test_window_at_large_frames(prec, gen):
     approximate score, cross and self, stays inside kappa (G_q + max G_ref);
     and queries at a window 10^4 times too narrow are caught by the run-time
     check (rows recomputed, outputs unchanged)."""
-    kind, natoms, extra = gen.split("-")
+    kind, natoms, extra = (gen.split("-") + [""])[:3]
     natoms = int(natoms)
     if kind == "metastable":
         allf = synth.metastable_trajectory(176, natoms, natoms + 3, n_centres=4,
