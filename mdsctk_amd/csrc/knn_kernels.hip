@@ -1057,53 +1057,69 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
 }
 
 // ---------------------------------------------------------------------------
-// exact FP64 score of one pair, cooperatively by a wave (lanes over atoms).
+// exact FP64 score of one pair, by a 16-lane group (lanes over atoms).
 
-// The query frame of a merge workgroup, staged once in LDS (raw float xyz).
-extern __shared__ F3 q_stage[];
+// A merge workgroup's query frame, staged once in LDS as its weighted
+// centred coordinates w_i (b_i - c_b) in FP64 (3 per atom).
+extern __shared__ double qw_stage[];
 
-// B_LDS: the query frame b is q_stage (the pointer argument is unused).
-// The arithmetic is the same either way, so both give the same bits.
-template <bool B_LDS>
-__device__ __forceinline__ double wave_exact_score_t(const float* __restrict__ a,
-                                                     const double* __restrict__ ca, double ga,
-                                                     const float* __restrict__ b,
-                                                     const double* __restrict__ cb, double gb,
-                                                     const float* __restrict__ w, int natoms,
-                                                     bool nofit) {
-  const int lane = threadIdx.x & 63;
-  double S[9];
+__device__ __forceinline__ double group_sum16(double v) {
 #pragma unroll
-  for (int i = 0; i < 9; ++i) S[i] = 0.0;
-  const double ca0 = ca[0], ca1 = ca[1], ca2 = ca[2];
-  const double cb0 = cb[0], cb1 = cb[1], cb2 = cb[2];
-  const F3* __restrict__ a3 = reinterpret_cast<const F3*>(a);
-  const F3* __restrict__ b3 = reinterpret_cast<const F3*>(b);
-#pragma unroll 2
-  for (int i = lane; i < natoms; i += 64) {
-    const double wi = (double)w[i];
-    const F3 av = a3[i];
-    const F3 bv = B_LDS ? q_stage[i] : b3[i];
-    const double ax = (double)av.x - ca0, ay = (double)av.y - ca1, az = (double)av.z - ca2;
-    const double bx = (double)bv.x - cb0, by = (double)bv.y - cb1, bz = (double)bv.z - cb2;
-    const double wx = wi * ax, wy = wi * ay, wz = wi * az;
-    S[0] += wx * bx; S[1] += wx * by; S[2] += wx * bz;
-    S[3] += wy * bx; S[4] += wy * by; S[5] += wy * bz;
-    S[6] += wz * bx; S[7] += wz * by; S[8] += wz * bz;
-  }
-#pragma unroll
-  for (int i = 0; i < 9; ++i) S[i] = wave_sum_d(S[i]);
-  const double lam = nofit ? (S[0] + S[4] + S[8]) : qcp_lambda_exact(S, ga, gb);
-  return fmax(ga + gb - 2.0 * lam, 0.0);
+  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
 }
 
-__device__ __forceinline__ double wave_exact_score(const float* __restrict__ a,
-                                                   const double* __restrict__ ca, double ga,
+// The weighted centred query coordinate w (x - c), formed the same way
+// wherever it is needed (staged or on the fly), so both give the same bits.
+__device__ __forceinline__ double weighted_centred(float x, double c, double w) {
+  return w * ((double)x - c);
+}
+
+// Exact FP64 score of the pair (reference frame a, query frame b) by the
+// 16 lanes of a group, lane j of the group taking atoms j, j + 16, ...:
+//   S = sum_i a_i (w_i (b_i - c_b))^T
+// with the reference frame uncentred (sum_i w_i (b_i - c_b) = 0, so S is
+// the weighted cross-covariance of the centred frames), then lambda_max of
+// the key matrix and s = G_a + G_b - 2 lambda_max.  Every lane of the group
+// returns the same value.  QS: the query's weighted centred coordinates
+// are in qw_stage (the pointer b is then unused).
+template <bool QS>
+__device__ __forceinline__ double group_rmsd_score(const float* __restrict__ a, double ga,
                                                    const float* __restrict__ b,
                                                    const double* __restrict__ cb, double gb,
                                                    const float* __restrict__ w, int natoms,
                                                    bool nofit) {
-  return wave_exact_score_t<false>(a, ca, ga, b, cb, gb, w, natoms, nofit);
+  const int sub = threadIdx.x & 15;
+  double S[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = 0.0;
+  const double cb0 = cb[0], cb1 = cb[1], cb2 = cb[2];
+  const F3* __restrict__ a3 = reinterpret_cast<const F3*>(a);
+  const F3* __restrict__ b3 = reinterpret_cast<const F3*>(b);
+#pragma unroll 8
+  for (int i = sub; i < natoms; i += 16) {
+    const F3 av = a3[i];
+    double bx, by, bz;
+    if constexpr (QS) {
+      bx = qw_stage[3 * i];
+      by = qw_stage[3 * i + 1];
+      bz = qw_stage[3 * i + 2];
+    } else {
+      const F3 bv = b3[i];
+      const double wi = (double)w[i];
+      bx = weighted_centred(bv.x, cb0, wi);
+      by = weighted_centred(bv.y, cb1, wi);
+      bz = weighted_centred(bv.z, cb2, wi);
+    }
+    const double ax = (double)av.x, ay = (double)av.y, az = (double)av.z;
+    S[0] = fma(ax, bx, S[0]); S[1] = fma(ax, by, S[1]); S[2] = fma(ax, bz, S[2]);
+    S[3] = fma(ay, bx, S[3]); S[4] = fma(ay, by, S[4]); S[5] = fma(ay, bz, S[5]);
+    S[6] = fma(az, bx, S[6]); S[7] = fma(az, by, S[7]); S[8] = fma(az, bz, S[8]);
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = group_sum16(S[i]);
+  const double lam = nofit ? (S[0] + S[4] + S[8]) : qcp_lambda_exact(S, ga, gb);
+  return fmax(ga + gb - 2.0 * lam, 0.0);
 }
 
 // Exact FP64 score of one pair of data vectors (a: reference, b: query),
@@ -1153,13 +1169,17 @@ struct PairSource {
   const double* faux;
 };
 
-template <bool B_LDS = false>
-__device__ __forceinline__ double wave_pair_score(const PairSource& P, int64_t r, int64_t q,
-                                                  bool nofit) {
-  if (P.metric == kMetricRMSD)
-    return wave_exact_score_t<B_LDS>(P.rxyz + r * (int64_t)P.natoms * 3, P.rcen + r * 3, P.rG[r],
-                                     P.fxyz + q * (int64_t)P.natoms * 3, P.fcen + q * 3, P.fG[q],
-                                     P.w, P.natoms, nofit);
+// RMSD: the exact score of reference r against query q by a 16-lane group.
+template <bool QS = false>
+__device__ __forceinline__ double group_pair_score(const PairSource& P, int64_t r, int64_t q,
+                                                   bool nofit) {
+  return group_rmsd_score<QS>(P.rxyz + r * (int64_t)P.natoms * 3, P.rG[r],
+                              P.fxyz + q * (int64_t)P.natoms * 3, P.fcen + q * 3, P.fG[q], P.w,
+                              P.natoms, nofit);
+}
+
+// Data vectors: the exact score of reference r against query q by a wave.
+__device__ __forceinline__ double wave_data_pair_score(const PairSource& P, int64_t r, int64_t q) {
   return wave_data_score(P.r64 + r * (int64_t)P.vdim, P.raux + r * 2, P.f64 + q * (int64_t)P.vdim,
                          P.faux + q * 2, P.vdim, P.metric);
 }
@@ -1213,10 +1233,11 @@ __device__ void block_bitonic_sort(double* s, int* idx, int n_pow2) {
 // Keys just above a row's cut that the merge re-scores to check the window.
 constexpr int kMergeGuard = 32;
 
-// STAGE: frames whose query frame fits kMergeStageAtoms are re-scored
-// against an LDS copy of it, so each candidate reads only its reference
-// frame from memory.
-constexpr int kMergeStageAtoms = 2048;
+// STAGE: for frames of up to kMergeStageAtoms atoms the query frame's
+// weighted centred coordinates are formed once, in FP64 in LDS (24 KB at
+// 1024 atoms), so each candidate reads only its reference frame from memory
+// and spends 3 conversions and 9 FMAs per atom.
+constexpr int kMergeStageAtoms = 1024;
 
 // CAND: the most candidates a row's window may hold (more: exact path).
 // The smaller instance leaves room for more workgroups per CU; the
@@ -1330,17 +1351,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
 
   if constexpr (STAGE) {
     const F3* fx = reinterpret_cast<const F3*>(P.src.fxyz) + q * (int64_t)P.src.natoms;
-    for (int i = tid; i < P.src.natoms; i += 256) q_stage[i] = fx[i];
+    const double c0 = P.src.fcen[q * 3], c1 = P.src.fcen[q * 3 + 1], c2 = P.src.fcen[q * 3 + 2];
+    for (int i = tid; i < P.src.natoms; i += 256) {
+      const F3 b = fx[i];
+      const double wi = (double)P.src.w[i];
+      qw_stage[3 * i] = weighted_centred(b.x, c0, wi);
+      qw_stage[3 * i + 1] = weighted_centred(b.y, c1, wi);
+      qw_stage[3 * i + 2] = weighted_centred(b.z, c2, wi);
+    }
     __syncthreads();
   }
-  for (int c = wave; c < nc + ng; c += 4) {
-    const int r = c < nc ? cand_idx[c] : guard_idx[c - nc];
-    const double s = wave_pair_score<STAGE>(P.src, r, q, P.nofit);
-    if (lane == 0) {
-      if (c < nc)
-        cand_s[c] = s;
-      else
-        guard_s[c - nc] = s;
+  auto put = [&](int c, double s) {
+    if (c < nc)
+      cand_s[c] = s;
+    else
+      guard_s[c - nc] = s;
+  };
+  if (P.src.metric == kMetricRMSD) {
+    // four pairs per wave, one per 16-lane group
+    const int grp = lane >> 4;
+    for (int c0 = 4 * wave; c0 < nc + ng; c0 += 16) {
+      const int c = c0 + grp;
+      if (c < nc + ng) {
+        const int r = c < nc ? cand_idx[c] : guard_idx[c - nc];
+        const double s = group_pair_score<STAGE>(P.src, r, q, P.nofit);
+        if ((lane & 15) == 0) put(c, s);
+      }
+    }
+  } else {
+    for (int c = wave; c < nc + ng; c += 4) {
+      const int r = c < nc ? cand_idx[c] : guard_idx[c - nc];
+      const double s = wave_data_pair_score(P.src, r, q);
+      if (lane == 0) put(c, s);
     }
   }
   __syncthreads();
@@ -1599,15 +1641,27 @@ __global__ __launch_bounds__(256) void exact_rows_kernel(ExactParams P) {
   const int64_t ri = blockIdx.y;
   const int64_t q = P.rows ? P.rows[ri] : P.row0 + ri;
   const int64_t r0 = (int64_t)blockIdx.x * kExactRefsPerWG;
-  for (int j = wave; j < kExactRefsPerWG; j += 4) {
-    const int64_t r = r0 + j;
-    if (r >= P.n_ref) break;
-    const double s = wave_pair_score(P.src, r, q, P.nofit);
-    if (lane == 0) {
-      if (P.out_full)
-        P.out_full[q * P.out_ld + r] = P.out_scale * sqrt(fmax(s, 0.0) / P.W);
-      else
-        P.srow[ri * P.n_ref + r] = s;
+  auto put = [&](int64_t r, double s) {
+    if (P.out_full)
+      P.out_full[q * P.out_ld + r] = P.out_scale * sqrt(fmax(s, 0.0) / P.W);
+    else
+      P.srow[ri * P.n_ref + r] = s;
+  };
+  if (P.src.metric == kMetricRMSD) {
+    // four pairs per wave, one per 16-lane group
+    const int grp = lane >> 4;
+    for (int j = 4 * wave + grp; j < kExactRefsPerWG; j += 16) {
+      const int64_t r = r0 + j;
+      if (r >= P.n_ref) break;
+      const double s = group_pair_score(P.src, r, q, P.nofit);
+      if ((lane & 15) == 0) put(r, s);
+    }
+  } else {
+    for (int j = wave; j < kExactRefsPerWG; j += 4) {
+      const int64_t r = r0 + j;
+      if (r >= P.n_ref) break;
+      const double s = wave_data_pair_score(P.src, r, q);
+      if (lane == 0) put(r, s);
     }
   }
 }
@@ -2114,7 +2168,7 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
   // k1 <= 256: windows of up to 1024 candidates (C3: 184 per query), half
   // the candidate LDS of the general instance
   const bool stage = w.metric == kMetricRMSD && ref.natoms <= kMergeStageAtoms;
-  const size_t dyn = stage ? (size_t)ref.natoms * sizeof(F3) : 0;
+  const size_t dyn = stage ? (size_t)ref.natoms * 3 * sizeof(double) : 0;
   const dim3 grid((unsigned)fit.n), block(256);
   if (k1 <= kMergeSmallK1) {
     if (stage)
