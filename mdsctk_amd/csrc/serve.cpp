@@ -14,50 +14,25 @@
 // Output values are independent of batch composition: each is the FP64
 // re-score of a certified candidate, sorted by (distance, index), computed by
 // the same per-pair code whatever rows share the batch.
+//
+// The queue, the batch packing and the serving thread's loop are in
+// serve_queue.h, free of device calls; this file is their HIP side
+// (HipServeDevice) and the C entry points.
 #include <algorithm>
 #include <atomic>
-#include <condition_variable>
 #include <cstring>
-#include <deque>
 #include <memory>
-#include <thread>
 
 #include "engine.h"
+#include "serve_queue.h"
 
 namespace mdsctk {
 
 namespace {
 
-struct Request {
-  int64_t id = 0;
-  const float* xyz = nullptr;
-  int64_t n = 0;
-  QueryShape q;
-  double* out_dist = nullptr;
-  int32_t* out_idx = nullptr;
-  int64_t next_row = 0;   // rows placed in batches so far
-  int64_t rows_done = 0;  // rows whose outputs are written
-  int status = MDSCTK_ERR_PENDING;
-  std::string err;
-  double t_submit = 0, t_admit = -1, t_done = -1;
-  int n_batches = 0;
-  int64_t fallback_rows = 0;
-  int64_t batch_frames = 0;
-};
-
-struct Chunk {
-  std::shared_ptr<Request> r;
-  int64_t row0 = 0;   // first row of the request in this batch
-  int64_t nrows = 0;
-  int64_t brow0 = 0;  // its first row in the batch
-};
-
-struct ServeSlot {
+// Device state of one batch slot.
+struct HipSlot {
   Slot s;
-  bool busy = false;
-  std::vector<Chunk> chunks;
-  QueryShape q;
-  int64_t n = 0;
   double* d_dist = nullptr;
   int* d_idx = nullptr;
   int* d_fb = nullptr;
@@ -73,24 +48,30 @@ struct ServeSlot {
   // launched), so nothing the executable may refer to is released while it
   // can still run
   hipGraph_t graph = nullptr;
-  int64_t limit = 0;  // frame limit of the batch in the slot
   // shape the workspace was last sized for
   int64_t sized_limit = 0;
   int sized_k_out = -1, sized_k1 = -1;
 };
 
-bool same_shape(const QueryShape& a, const QueryShape& b) {
-  return a.k_out == b.k_out && a.k1 == b.k1 && a.rank_off == b.rank_off && a.nofit == b.nofit;
-}
-
 }  // namespace
 
-class Server {
+// The HIP side of the serving queue (the Device of serve_queue.h): a batch
+// is H2D of its frames, prep, contraction and merge on the slot's stream
+// (replayed as a graph), D2H of the outputs into pinned buffers.
+class HipServeDevice {
  public:
-  explicit Server(mdsctk_knn_ctx* c) : c_(c) {}
+  using Shape = QueryShape;
+  using SlotState = HipSlot;
+  using Batch = ServeBatch<QueryShape>;
 
-  int start() {
-    slots_.resize(c_->serve_slots);
+  explicit HipServeDevice(mdsctk_knn_ctx* c) : c_(c) {}
+
+  static bool same_shape(const QueryShape& a, const QueryShape& b) {
+    return a.k_out == b.k_out && a.k1 == b.k1 && a.rank_off == b.rank_off && a.nofit == b.nofit;
+  }
+
+  int init(int n_slots) {
+    slots_.resize(n_slots);
     for (auto& sl : slots_) {
       int rc = slot_init(c_, sl.s);
       if (rc) return rc;
@@ -101,17 +82,10 @@ class Server {
       std::lock_guard<std::mutex> g(c_->mem_mu);
       avail_ = c_->budget ? (c_->budget > c_->held ? c_->budget - c_->held : 0) : 0;
     }
-    th_ = std::thread([this] { run(); });
     return 0;
   }
 
-  ~Server() {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      stop_ = true;
-    }
-    cv_work_.notify_all();
-    if (th_.joinable()) th_.join();
+  void release() {
     for (auto& sl : slots_) {
       if (sl.s.st) (void)hipStreamSynchronize(sl.s.st);
       if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
@@ -120,73 +94,132 @@ class Server {
       if (sl.done) (void)hipEventDestroy(sl.done);
       free_host(sl);
     }
+    slots_.clear();
   }
 
-  int submit(const float* xyz, int64_t n, const QueryShape& q, double* od, int32_t* oi,
-             int64_t* id) {
-    auto r = std::make_shared<Request>();
-    r->xyz = xyz;
-    r->n = n;
-    r->q = q;
-    r->out_dist = od;
-    r->out_idx = oi;
-    r->t_submit = clock_ms(c_);
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      r->id = next_id_++;
-      *id = r->id;
-      live_[r->id] = r;
-      if (n == 0 || q.k_out == 0) {
-        r->status = q.k_out;
-        r->t_admit = r->t_done = r->t_submit;
-      } else {
-        pending_.push_back(r);
-      }
+  std::vector<HipSlot*> slot_states() {
+    std::vector<HipSlot*> v;
+    for (auto& sl : slots_) v.push_back(&sl);
+    return v;
+  }
+
+  double now() { return clock_ms(c_); }
+
+  void thread_init() {
+    (void)hipSetDevice(c_->device);
+    set_error_sink(&err_);
+  }
+
+  const std::string& error_text() { return err_; }
+  int64_t ref_frames() { return c_->ref.n; }
+  int64_t graph_launches() const { return graph_launches_.load(); }
+  int64_t graph_builds() const { return graph_builds_.load(); }
+
+  // Largest batch (frames) whose workspace fits, for every slot at once,
+  // in the HBM budget left when serving started.
+  int64_t frame_limit(const QueryShape& q) const {
+    int64_t f = std::max<int64_t>(1, c_->serve_max_frames);
+    if (!avail_) return f;
+    while (f > 32 && (double)slot_bytes(f, q) * slots_.size() > 0.9 * (double)avail_) f /= 2;
+    return f;
+  }
+
+  // Enqueue the whole batch on the slot.
+  int launch(HipSlot& sl, const Batch& b) {
+    Slot& s = sl.s;
+    const QueryShape& q = b.q;
+    const int natoms = c_->ref.natoms;
+    int rc;
+    if (sl.sized_limit < b.limit || sl.sized_k_out != q.k_out || sl.sized_k1 != q.k1)
+      if ((rc = presize(sl, q, b.limit))) return rc;
+    if ((rc = slot_frames(c_, s, b.n))) return rc;
+    const size_t elems = (size_t)b.n * q.k_out;
+    if ((rc = ensure(c_, &s.ws_out, &s.ws_out_bytes, elems * (sizeof(double) + sizeof(int)),
+                     "serve outputs")))
+      return rc;
+    sl.d_dist = (double*)s.ws_out;
+    sl.d_idx = (int*)((char*)s.ws_out + elems * sizeof(double));
+    if ((rc = ensure_host(sl, elems, (size_t)b.n))) return rc;
+    const size_t frame_bytes = (size_t)natoms * 3 * sizeof(float);
+    for (const auto& ch : b.chunks) {
+      hipError_t e = hipMemcpyAsync((char*)s.fit.xyz + ch.brow0 * frame_bytes,
+                                    ch.r->xyz + ch.row0 * natoms * 3, ch.nrows * frame_bytes,
+                                    hipMemcpyHostToDevice, s.st);
+      if (e != hipSuccess) return hip_fail(c_, e, "serve: query upload");
     }
-    cv_work_.notify_one();
+    if (q.fused) {
+      FusedPlan plan;
+      if ((rc = plan_fused(c_, s, q, &plan))) return rc;
+      if (c_->serve_share_cus && slots_.size() > 1)
+        plan.grid = std::max(8, (plan.grid / (int)slots_.size()) & ~7);
+      if (c_->serve_graphs) {
+        if ((rc = replay_graph(sl, q, plan))) return rc;
+      } else {
+        if ((rc = prep_slot(c_, s))) return rc;
+        if ((rc = enqueue_fused(c_, s, q, plan, sl.d_dist, sl.d_idx, false))) return rc;
+      }
+      sl.d_fb = plan.d_fallback;
+      if (hipMemcpyAsync(sl.h_fb, sl.d_fb, b.n * sizeof(int), hipMemcpyDeviceToHost, s.st) !=
+          hipSuccess)
+        return fail(c_, MDSCTK_ERR_HIP, "serve: flag copy");
+    } else {
+      if ((rc = prep_slot(c_, s))) return rc;
+      if ((rc = exact_rows_select(c_, s, s.fit, nullptr, (int)b.n, q, sl.d_dist, sl.d_idx)))
+        return rc;
+      std::memset(sl.h_fb, 0, b.n * sizeof(int));
+    }
+    if (hipMemcpyAsync(sl.h_dist, sl.d_dist, elems * sizeof(double), hipMemcpyDeviceToHost,
+                       s.st) != hipSuccess ||
+        hipMemcpyAsync(sl.h_idx, sl.d_idx, elems * sizeof(int), hipMemcpyDeviceToHost, s.st) !=
+            hipSuccess ||
+        hipEventRecord(sl.done, s.st) != hipSuccess)
+      return fail(c_, MDSCTK_ERR_HIP, "serve: output copy");
     return 0;
   }
 
-  int poll(int64_t id, mdsctk_knn_request_info* info, bool wait, std::string* err) {
-    std::unique_lock<std::mutex> lk(mu_);
-    auto it = live_.find(id);
-    if (it == live_.end()) {
-      *err = "request id unknown (never submitted or already retired)";
-      return MDSCTK_ERR_STATE;
-    }
-    std::shared_ptr<Request> r = it->second;
-    if (wait) cv_done_.wait(lk, [&] { return r->status != MDSCTK_ERR_PENDING; });
-    if (info) {
-      info->status = r->status;
-      info->n_batches = r->n_batches;
-      info->n_fallback_rows = r->fallback_rows;
-      info->t_submit_ms = r->t_submit;
-      info->t_admit_ms = r->t_admit;
-      info->t_done_ms = r->t_done;
-      info->batch_frames = r->batch_frames;
-    }
-    if (r->status != MDSCTK_ERR_PENDING) {
-      live_.erase(it);
-      if (r->status < 0) *err = r->err;
-    }
-    return r->status;
+  int poll(HipSlot& sl) {
+    const hipError_t e = hipEventQuery(sl.done);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    return hip_fail(c_, e, "serve: batch");
   }
 
-  void stats(mdsctk_knn_serve_stats* out) {
-    std::lock_guard<std::mutex> g(mu_);
-    *out = st_;
-    out->slots = (int32_t)slots_.size();
-    out->busy_ms = busy_ms_ + (busy_since_ >= 0 ? clock_ms(c_) - busy_since_ : 0.0);
-    out->graph_launches = graph_launches_.load();
-    out->graph_builds = graph_builds_.load();
-  }
+  void drain(HipSlot& sl) { (void)hipStreamSynchronize(sl.s.st); }
 
-  bool idle() {
-    std::lock_guard<std::mutex> g(mu_);
-    if (!pending_.empty()) return false;
-    for (auto& kv : live_)
-      if (kv.second->status == MDSCTK_ERR_PENDING) return false;
-    return true;
+  // The slot's work has finished.  Rows the merge could not certify take the
+  // exact path; then outputs go to the callers' buffers.
+  int finish(HipSlot& sl, const Batch& b, std::vector<int64_t>* fb_per_chunk) {
+    Slot& s = sl.s;
+    const QueryShape& q = b.q;
+    std::vector<int> rows;
+    for (int64_t i = 0; i < b.n; ++i)
+      if (sl.h_fb[i]) rows.push_back((int)i);
+    const size_t elems = (size_t)b.n * q.k_out;
+    if (!rows.empty()) {
+      int rc = exact_rows_select(c_, s, s.fit, rows.data(), (int)rows.size(), q, sl.d_dist,
+                                 sl.d_idx);
+      if (rc) return rc;
+      if (hipMemcpyAsync(sl.h_dist, sl.d_dist, elems * sizeof(double), hipMemcpyDeviceToHost,
+                         s.st) != hipSuccess ||
+          hipMemcpyAsync(sl.h_idx, sl.d_idx, elems * sizeof(int), hipMemcpyDeviceToHost, s.st) !=
+              hipSuccess ||
+          hipStreamSynchronize(s.st) != hipSuccess)
+        return fail(c_, MDSCTK_ERR_HIP, "serve: output copy");
+    }
+    fb_per_chunk->assign(b.chunks.size(), 0);
+    for (size_t j = 0; j < b.chunks.size(); ++j) {
+      const auto& ch = b.chunks[j];
+      // A request that already failed in another batch has been handed back
+      // to its caller: write nothing more into its buffers.  (Only the
+      // serving thread changes a request's status once it is queued.)
+      if (ch.r->status != MDSCTK_ERR_PENDING) continue;
+      const size_t off_b = (size_t)ch.brow0 * q.k_out, off_r = (size_t)ch.row0 * q.k_out;
+      const size_t cnt = (size_t)ch.nrows * q.k_out;
+      std::memcpy(ch.r->out_dist + off_r, sl.h_dist + off_b, cnt * sizeof(double));
+      std::memcpy(ch.r->out_idx + off_r, sl.h_idx + off_b, cnt * sizeof(int));
+      for (int64_t i = 0; i < ch.nrows; ++i) (*fb_per_chunk)[j] += sl.h_fb[ch.brow0 + i] ? 1 : 0;
+    }
+    return 0;
   }
 
  private:
@@ -218,20 +251,10 @@ class Server {
     return (size_t)b;
   }
 
-  // Largest batch (frames) whose workspace fits, for every slot at once,
-  // in the HBM budget left when serving started.
-  int64_t frame_limit(const QueryShape& q) const {
-    int64_t f = std::max<int64_t>(1, c_->serve_max_frames);
-    if (!avail_) return f;
-    while (f > 32 && (double)slot_bytes(f, q) * slots_.size() > 0.9 * (double)avail_) f /= 2;
-    return f;
-  }
-
   // Size the slot's buffers once for its largest batch of this shape, so
   // that steady-state batches allocate nothing.
-  int presize(ServeSlot& sl, int64_t limit) {
+  int presize(HipSlot& sl, const QueryShape& q, int64_t limit) {
     Slot& s = sl.s;
-    const QueryShape& q = sl.q;
     int rc;
     if ((rc = slot_frames(c_, s, limit))) return rc;
     if (q.fused) {
@@ -252,7 +275,7 @@ class Server {
     return 0;
   }
 
-  static void free_host(ServeSlot& sl) {
+  static void free_host(HipSlot& sl) {
     if (sl.h_dist) (void)hipHostFree(sl.h_dist);
     if (sl.h_idx) (void)hipHostFree(sl.h_idx);
     if (sl.h_fb) (void)hipHostFree(sl.h_fb);
@@ -262,7 +285,7 @@ class Server {
     sl.h_elems = sl.h_rows = 0;
   }
 
-  int ensure_host(ServeSlot& sl, size_t elems, size_t rows) {
+  int ensure_host(HipSlot& sl, size_t elems, size_t rows) {
     if (elems > sl.h_elems) {
       if (sl.h_dist) (void)hipHostFree(sl.h_dist);
       if (sl.h_idx) (void)hipHostFree(sl.h_idx);
@@ -286,92 +309,6 @@ class Server {
     return 0;
   }
 
-  // Under mu_: pack pending requests into the slot.  Returns false when
-  // nothing is pending.
-  bool form_batch(ServeSlot& sl) {
-    if (pending_.empty()) return false;
-    sl.chunks.clear();
-    sl.q = pending_.front()->q;
-    const int64_t limit = frame_limit(sl.q);
-    int64_t n = 0;
-    const double now = clock_ms(c_);
-    for (auto it = pending_.begin(); it != pending_.end() && n < limit;) {
-      Request& r = **it;
-      if (!same_shape(r.q, sl.q)) {
-        ++it;
-        continue;
-      }
-      const int64_t take = std::min(r.n - r.next_row, limit - n);
-      sl.chunks.push_back(Chunk{*it, r.next_row, take, n});
-      if (r.t_admit < 0) r.t_admit = now;
-      ++r.n_batches;
-      r.next_row += take;
-      n += take;
-      if (r.next_row == r.n)
-        it = pending_.erase(it);
-      else
-        break;  // the batch is full; the rest of this request goes next
-    }
-    sl.n = n;
-    sl.limit = limit;
-    st_.max_batch_frames = std::max(st_.max_batch_frames, n);
-    st_.slot_frames = (int32_t)limit;
-    return true;
-  }
-
-  // Serving thread, mu_ not held: enqueue the whole batch on the slot.
-  int launch(ServeSlot& sl) {
-    Slot& s = sl.s;
-    const QueryShape& q = sl.q;
-    const int natoms = c_->ref.natoms;
-    int rc;
-    if (sl.sized_limit < sl.limit || sl.sized_k_out != q.k_out || sl.sized_k1 != q.k1)
-      if ((rc = presize(sl, sl.limit))) return rc;
-    if ((rc = slot_frames(c_, s, sl.n))) return rc;
-    const size_t elems = (size_t)sl.n * q.k_out;
-    if ((rc = ensure(c_, &s.ws_out, &s.ws_out_bytes, elems * (sizeof(double) + sizeof(int)),
-                     "serve outputs")))
-      return rc;
-    sl.d_dist = (double*)s.ws_out;
-    sl.d_idx = (int*)((char*)s.ws_out + elems * sizeof(double));
-    if ((rc = ensure_host(sl, elems, (size_t)sl.n))) return rc;
-    const size_t frame_bytes = (size_t)natoms * 3 * sizeof(float);
-    for (const Chunk& ch : sl.chunks) {
-      hipError_t e = hipMemcpyAsync((char*)s.fit.xyz + ch.brow0 * frame_bytes,
-                                    ch.r->xyz + ch.row0 * natoms * 3, ch.nrows * frame_bytes,
-                                    hipMemcpyHostToDevice, s.st);
-      if (e != hipSuccess) return hip_fail(c_, e, "serve: query upload");
-    }
-    if (q.fused) {
-      FusedPlan plan;
-      if ((rc = plan_fused(c_, s, q, &plan))) return rc;
-      if (c_->serve_share_cus && slots_.size() > 1)
-        plan.grid = std::max(8, (plan.grid / (int)slots_.size()) & ~7);
-      if (c_->serve_graphs) {
-        if ((rc = replay_graph(sl, plan))) return rc;
-      } else {
-        if ((rc = prep_slot(c_, s))) return rc;
-        if ((rc = enqueue_fused(c_, s, q, plan, sl.d_dist, sl.d_idx, false))) return rc;
-      }
-      sl.d_fb = plan.d_fallback;
-      if (hipMemcpyAsync(sl.h_fb, sl.d_fb, sl.n * sizeof(int), hipMemcpyDeviceToHost, s.st) !=
-          hipSuccess)
-        return fail(c_, MDSCTK_ERR_HIP, "serve: flag copy");
-    } else {
-      if ((rc = prep_slot(c_, s))) return rc;
-      if ((rc = exact_rows_select(c_, s, s.fit, nullptr, (int)sl.n, q, sl.d_dist, sl.d_idx)))
-        return rc;
-      std::memset(sl.h_fb, 0, sl.n * sizeof(int));
-    }
-    if (hipMemcpyAsync(sl.h_dist, sl.d_dist, elems * sizeof(double), hipMemcpyDeviceToHost,
-                       s.st) != hipSuccess ||
-        hipMemcpyAsync(sl.h_idx, sl.d_idx, elems * sizeof(int), hipMemcpyDeviceToHost, s.st) !=
-            hipSuccess ||
-        hipEventRecord(sl.done, s.st) != hipSuccess)
-      return fail(c_, MDSCTK_ERR_HIP, "serve: output copy");
-    return 0;
-  }
-
   // The batch's kernels (prep, list init, window, contraction, merge) are
   // captured from the slot's stream into a graph.  The slot keeps one
   // executable graph and updates it in place with each new capture (same
@@ -382,12 +319,12 @@ class Server {
   // the executable is only ever launched while the graph it came from
   // exists (a tracer reading the launched kernels' node data at launch time
   // then never reads released memory).
-  int replay_graph(ServeSlot& sl, const FusedPlan& plan) {
+  int replay_graph(HipSlot& sl, const QueryShape& q, const FusedPlan& plan) {
     Slot& s = sl.s;
     hipError_t e = hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(c_, e, "serve: graph capture");
     int rc = prep_slot(c_, s);
-    if (!rc) rc = enqueue_fused(c_, s, sl.q, plan, sl.d_dist, sl.d_idx, false);
+    if (!rc) rc = enqueue_fused(c_, s, q, plan, sl.d_dist, sl.d_idx, false);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s.st, &g);
     if (rc || e != hipSuccess) {
@@ -424,169 +361,51 @@ class Server {
     return 0;
   }
 
-  // Serving thread, mu_ not held: the slot's work has finished.  Rows the
-  // merge could not certify take the exact path; then outputs go to the
-  // callers' buffers.
-  int finish(ServeSlot& sl, std::vector<int64_t>* fb_per_chunk) {
-    Slot& s = sl.s;
-    const QueryShape& q = sl.q;
-    std::vector<int> rows;
-    for (int64_t i = 0; i < sl.n; ++i)
-      if (sl.h_fb[i]) rows.push_back((int)i);
-    const size_t elems = (size_t)sl.n * q.k_out;
-    if (!rows.empty()) {
-      int rc = exact_rows_select(c_, s, s.fit, rows.data(), (int)rows.size(), q, sl.d_dist,
-                                 sl.d_idx);
-      if (rc) return rc;
-      if (hipMemcpyAsync(sl.h_dist, sl.d_dist, elems * sizeof(double), hipMemcpyDeviceToHost,
-                         s.st) != hipSuccess ||
-          hipMemcpyAsync(sl.h_idx, sl.d_idx, elems * sizeof(int), hipMemcpyDeviceToHost, s.st) !=
-              hipSuccess ||
-          hipStreamSynchronize(s.st) != hipSuccess)
-        return fail(c_, MDSCTK_ERR_HIP, "serve: output copy");
+  mdsctk_knn_ctx* c_;
+  std::vector<HipSlot> slots_;
+  size_t avail_ = 0;  // HBM budget left when serving started (0 = no budget)
+  std::atomic<int64_t> graph_launches_{0}, graph_builds_{0};
+  std::string err_;  // error text of the serving thread's last failure
+};
+
+class Server {
+ public:
+  explicit Server(mdsctk_knn_ctx* c) : dev_(c) {}
+
+  int start(int n_slots) {
+    int rc = dev_.init(n_slots);
+    if (rc) {
+      dev_.release();
+      return rc;
     }
-    fb_per_chunk->assign(sl.chunks.size(), 0);
-    for (size_t j = 0; j < sl.chunks.size(); ++j) {
-      const Chunk& ch = sl.chunks[j];
-      // A request that already failed in another batch has been handed back
-      // to its caller: write nothing more into its buffers.  (Only this
-      // thread changes a request's status once it is queued.)
-      if (ch.r->status != MDSCTK_ERR_PENDING) continue;
-      const size_t off_b = (size_t)ch.brow0 * q.k_out, off_r = (size_t)ch.row0 * q.k_out;
-      const size_t cnt = (size_t)ch.nrows * q.k_out;
-      std::memcpy(ch.r->out_dist + off_r, sl.h_dist + off_b, cnt * sizeof(double));
-      std::memcpy(ch.r->out_idx + off_r, sl.h_idx + off_b, cnt * sizeof(int));
-      for (int64_t i = 0; i < ch.nrows; ++i) (*fb_per_chunk)[j] += sl.h_fb[ch.brow0 + i] ? 1 : 0;
-    }
+    q_ = std::make_unique<ServeQueue<HipServeDevice>>(dev_, dev_.slot_states());
+    q_->start();
     return 0;
   }
 
-  // Under mu_: record the outcome of a slot's batch.
-  void retire(ServeSlot& sl, int rc, const std::vector<int64_t>& fb_per_chunk) {
-    const double now = clock_ms(c_);
-    for (size_t j = 0; j < sl.chunks.size(); ++j) {
-      Request& r = *sl.chunks[j].r;
-      r.batch_frames = std::max(r.batch_frames, sl.n);
-      if (r.status != MDSCTK_ERR_PENDING) continue;  // already failed
-      if (rc) {
-        r.status = rc;
-        r.err = err_;
-        r.t_done = now;
-        if (r.next_row < r.n) {
-          r.next_row = r.n;  // place no further rows
-          pending_.erase(std::remove(pending_.begin(), pending_.end(), sl.chunks[j].r),
-                         pending_.end());
-        }
-        continue;
-      }
-      r.rows_done += sl.chunks[j].nrows;
-      r.fallback_rows += j < fb_per_chunk.size() ? fb_per_chunk[j] : 0;
-      if (r.rows_done == r.n) {
-        r.status = r.q.k_out;
-        r.t_done = now;
-        ++st_.requests_done;
-      }
-    }
-    if (!rc) {
-      ++st_.batches;
-      st_.frames += sl.n;
-      st_.pairs += sl.n * c_->ref.n;
-      for (int64_t v : fb_per_chunk) st_.fallback_rows += v;
-    }
-    sl.chunks.clear();
-    sl.busy = false;
-    cv_done_.notify_all();
+  ~Server() {
+    if (q_) q_->stop();
+    q_.reset();
+    dev_.release();
   }
 
-  void note_busy_change() {
-    bool any = false;
-    for (auto& sl : slots_) any |= sl.busy;
-    if (any && busy_since_ < 0) busy_since_ = clock_ms(c_);
-    if (!any && busy_since_ >= 0) {
-      busy_ms_ += clock_ms(c_) - busy_since_;
-      busy_since_ = -1;
-    }
+  int submit(const float* xyz, int64_t n, const QueryShape& q, double* od, int32_t* oi,
+             int64_t* id) {
+    return q_->submit(xyz, n, q, od, oi, id);
   }
-
-  void run() {
-    (void)hipSetDevice(c_->device);
-    set_error_sink(&err_);
-    std::unique_lock<std::mutex> lk(mu_);
-    for (;;) {
-      bool any_busy = false;
-      for (auto& sl : slots_) {
-        if (!sl.busy) continue;
-        lk.unlock();
-        const hipError_t e = hipEventQuery(sl.done);
-        int rc = 0;
-        std::vector<int64_t> fb;
-        if (e == hipSuccess)
-          rc = finish(sl, &fb);
-        else if (e != hipErrorNotReady)
-          rc = hip_fail(c_, e, "serve: batch");
-        lk.lock();
-        if (e == hipErrorNotReady) {
-          any_busy = true;
-          continue;
-        }
-        retire(sl, rc, fb);
-        note_busy_change();
-      }
-      if (stop_) {
-        if (any_busy) {
-          cv_work_.wait_for(lk, std::chrono::microseconds(50));
-          continue;
-        }
-        // Requests never started fail; callers are told the context closed.
-        for (auto& r : pending_) {
-          if (r->status == MDSCTK_ERR_PENDING) {
-            r->status = MDSCTK_ERR_STATE;
-            r->err = "context destroyed or reference set replaced";
-          }
-        }
-        pending_.clear();
-        cv_done_.notify_all();
-        return;
-      }
-      bool launched = false;
-      for (auto& sl : slots_) {
-        if (sl.busy || !form_batch(sl)) continue;
-        sl.busy = true;
-        note_busy_change();
-        lk.unlock();
-        const int rc = launch(sl);
-        if (rc) (void)hipStreamSynchronize(sl.s.st);
-        lk.lock();
-        if (rc) {
-          retire(sl, rc, {});
-          note_busy_change();
-        } else {
-          any_busy = true;
-          launched = true;
-        }
-      }
-      if (launched) continue;
-      if (any_busy)
-        cv_work_.wait_for(lk, std::chrono::microseconds(20));
-      else
-        cv_work_.wait(lk, [&] { return stop_ || !pending_.empty(); });
-    }
+  int poll(int64_t id, mdsctk_knn_request_info* info, bool wait, std::string* err) {
+    return q_->poll(id, info, wait, err);
   }
+  void stats(mdsctk_knn_serve_stats* out) {
+    q_->stats(out);
+    out->graph_launches = dev_.graph_launches();
+    out->graph_builds = dev_.graph_builds();
+  }
+  bool idle() { return q_->idle(); }
 
-  mdsctk_knn_ctx* c_;
-  std::mutex mu_;
-  std::condition_variable cv_work_, cv_done_;
-  std::deque<std::shared_ptr<Request>> pending_;
-  std::unordered_map<int64_t, std::shared_ptr<Request>> live_;
-  int64_t next_id_ = 1;
-  bool stop_ = false;
-  std::vector<ServeSlot> slots_;
-  size_t avail_ = 0;  // HBM budget left when serving started (0 = no budget)
-  mdsctk_knn_serve_stats st_ = {};
-  double busy_ms_ = 0, busy_since_ = -1;
-  std::atomic<int64_t> graph_launches_{0}, graph_builds_{0};
-  std::string err_;  // error text of the serving thread's last failure
-  std::thread th_;
+ private:
+  HipServeDevice dev_;
+  std::unique_ptr<ServeQueue<HipServeDevice>> q_;
 };
 
 int server_submit(mdsctk_knn_ctx* c, const float* xyz, int64_t n_fit, int32_t k, uint32_t flags,
@@ -608,7 +427,7 @@ int server_submit(mdsctk_knn_ctx* c, const float* xyz, int64_t n_fit, int32_t k,
   if (rc) return rc;
   if (!c->server) {
     auto* s = new Server(c);
-    rc = s->start();
+    rc = s->start(c->serve_slots);
     if (rc) {
       delete s;
       return rc;
