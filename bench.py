@@ -116,6 +116,37 @@ def time_fused(ctx, hip, n_fit, k, steps, warmup, first=None):
     return wall, stats, dist, idx
 
 
+def run_c4_batches(knn, synth, prec, args):
+    """The serving form's batch kernels without the serving thread: the
+    resident 100,000 x 500-atom set queried synchronously with batches of 1,
+    1000 and 4096 frames (k = 50), args.c4_batches times each.  Each call
+    runs the same prep, contraction and merge kernels a served batch of that
+    size replays; the per-size stage times come from the library's HIP
+    events, and a kernel trace of this run gives their per-kernel times."""
+    n_res, na_res, kq = 100_000, 500, 50
+    res = synth.trajectory(n_res, na_res, args.seed + 7)
+    w = synth.masses(na_res)
+    pool = synth.trajectory(4096, na_res, args.seed + 8, base_seed=args.seed + 7)
+    out = {"workload": f"resident {n_res} x {na_res} atoms, synchronous batches, k={kq}",
+           "by_batch_frames": {}}
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(res, w)
+        for nb in (1, 1000, 4096):
+            ctx.query(pool[:nb], kq)  # warm-up: sizes the workspace
+            st = []
+            for _ in range(args.c4_batches):
+                ctx.query(pool[:nb], kq)
+                st.append(ctx.stats())
+            out["by_batch_frames"][str(nb)] = {
+                "calls": args.c4_batches,
+                "ms_total": mean_stat(st, "ms_total"),
+                "ms_contract": mean_stat(st, "ms_contract"),
+                "ms_merge": mean_stat(st, "ms_merge"),
+                "fallback_rows": int(sum(x["n_fallback_rows"] for x in st))}
+            log(f"c4 batch {nb} frames: {out['by_batch_frames'][str(nb)]}")
+    return out
+
+
 def run_serving(knn, synth, oracle, prec, args):
     """Serving form: a resident 100,000 x 500-atom reference set; requests of
     log-uniform 1..4096 frames arrive as a Poisson process; each is submitted
@@ -475,6 +506,10 @@ def main():
     ap.add_argument("--cli", type=int, default=0,
                     help="also time the drop-in knn_rms executable end to end on the workload "
                          "(writes a 1.6 GB XTC to /tmp first; 0 = skip)")
+    ap.add_argument("--c4-batches", type=int, default=0, metavar="N",
+                    help="only run the serving form's batch sizes (1, 1000, 4096 frames against "
+                         "the 100k x 500 set) N times each as synchronous queries, print their "
+                         "stage times and exit (for a kernel trace of the served batch kernels)")
     ap.add_argument("--cpu-rows", type=int, default=128, help="fit rows of the CPU baseline sample (0 = skip)")
     ap.add_argument("--check-rows", type=int, default=8, help="rows checked against the FP64 oracle")
     ap.add_argument("--seed", type=int, default=2026)
@@ -498,6 +533,12 @@ def main():
 
     if args.traffic_child:
         traffic_child(args)
+        return
+
+    if args.c4_batches > 0:
+        from mdsctk_amd import knn, synth
+        print(json.dumps({"c4_batches": run_c4_batches(knn, synth, args.precision, args)}),
+              flush=True)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
