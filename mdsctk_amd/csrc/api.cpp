@@ -629,7 +629,26 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
   const int lm_pairs = (ref_pairs + lstride - 1) / lstride;
   const int nlab = lm_pairs * 2 * kTile;
   const size_t ent_bytes = (size_t)n * nl * cap * sizeof(uint2);
+  // Canonical form (fitted scores of frames, n >= 512): the main pass runs
+  // over a copy of the set in landmark order, every frame in its principal
+  // axes.  Frames near each other in the order are near each other in
+  // space, so most 32 x 32 tiles hold only far pairs, which the
+  // contraction's lambda_max bound rejects without the QCP; the axes make
+  // that bound nearly exact for similar frames.  The copy: operands (both
+  // dither sides), centroids, G, G~, rotations, windows, thresholds.
+  const bool canon = !q.nofit && c->w.metric == kMetricRMSD && n >= 512;
+  const bool dith = c->ref.op_q != nullptr;
+  const size_t rows = (size_t)c->ref.n_tiles * kTile;
+  const size_t opb = operand_bytes(c->ref.prec, c->ref.n_tiles, c->ref.n_groups);
+  const size_t perm_bytes =
+      canon ? Carve::round(rows * 3 * sizeof(double)) + Carve::round(rows * sizeof(double)) * 2 +
+                  (dith ? Carve::round(rows * sizeof(double)) + Carve::round(opb) : 0) +
+                  Carve::round(rows * 9 * sizeof(double)) + Carve::round(opb) +
+                  Carve::round((size_t)n * sizeof(float)) +
+                  Carve::round((size_t)n * sizeof(unsigned)) + 512
+            : 0;
   const size_t mir_bytes = (size_t)n * c->mirror_cap * sizeof(uint2);
+  const size_t sym_bytes = Carve::round(mir_bytes) + perm_bytes;
   const size_t small = (size_t)n * nl * (sizeof(int) + sizeof(float)) +
                        (size_t)n * 7 * sizeof(int) + (size_t)nlab * sizeof(int) + 1024;
   // Room: the free device memory plus what the buffers already hold, less a
@@ -637,14 +656,14 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
   size_t free_b = 0, total_b = 0;
   HIP_TRY(c, hipMemGetInfo(&free_b, &total_b));
   const size_t have = s.ws_sym_bytes + s.ws_lists_bytes + s.ws_small_bytes;
-  const size_t need = ent_bytes + mir_bytes + small;
+  const size_t need = ent_bytes + sym_bytes + small;
   const size_t reserve = size_t(2) << 30;
   if (need + reserve > free_b + have) return 1;
   {
     std::lock_guard<std::mutex> g(c->mem_mu);
     if (c->budget && c->held - std::min(c->held, have) + need > c->budget) return 1;
   }
-  int rc = ensure(c, &s.ws_sym, &s.ws_sym_bytes, mir_bytes, "mirrored lists");
+  int rc = ensure(c, &s.ws_sym, &s.ws_sym_bytes, sym_bytes, "mirrored lists");
   if (!rc) rc = ensure(c, &s.ws_lists, &s.ws_lists_bytes, ent_bytes, "candidate lists");
   if (!rc) rc = ensure(c, &s.ws_small, &s.ws_small_bytes, small, "list headers");
   if (rc == MDSCTK_ERR_NOMEM) {  // the list path instead, without the mirrored lists
@@ -686,18 +705,56 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
   HIP_TRY(c, launch_contract(c->ref, fit, LL, eps2, q.k1, lstride, q.nofit, nullptr, grid, s.st,
                              nullptr, 1));
   HIP_TRY(c, launch_landmark_cap(LL, n, q.k1, eps2, lstride, L.thr, label, s.st));
-  HIP_TRY(c, launch_init_lists(L, n, s.st, false));
-  HIP_TRY(c, launch_mirror_init(M, n, s.st));
-  HIP_TRY(c, hipEventRecord(s.ev[6], s.st));
-  HIP_TRY(c, launch_contract(c->ref, fit, L, eps2, q.k1, phases, q.nofit, nullptr, grid, s.st, &M));
-  HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
-  HIP_TRY(c, launch_mirror_sum(M, n, msum, s.st));
-  // merge rows grouped by nearest landmark (it pays where candidate frames
-  // no longer fit the caches)
+  // rows grouped by nearest landmark: the merge takes them in this order (it
+  // pays where candidate frames no longer fit the caches), and the canonical
+  // form contracts the frames in it
   const bool ordered = n >= 512;
   if (ordered) HIP_TRY(c, launch_row_order(label, n, nlab, counts, order, s.st));
-  HIP_TRY(c, launch_merge(c->ref, fit, c->w, L, eps2, q.k1, q.k_out, q.rank_off, q.nofit, d_dist,
-                          d_idx, d_fallback, s.n_cand, ordered ? order : nullptr, s.st, &M));
+  // the main pass's frames, windows and thresholds
+  DevFrames mref = c->ref, mfit = fit;
+  Lists LM = L;
+  float* meps2 = eps2;
+  if (canon) {
+    Carve cp;
+    cp.base = (char*)s.ws_sym + Carve::round(mir_bytes);
+    cp.size = perm_bytes;
+    mref.xyz = nullptr;
+    mref.owns_xyz = false;
+    mref.cen = (double*)cp.take(rows * 3 * sizeof(double));
+    mref.G = (double*)cp.take(rows * sizeof(double));
+    mref.Gt = (double*)cp.take(rows * sizeof(double));
+    mref.op = (uint4*)cp.take(opb);
+    mref.op_q = dith ? (uint4*)cp.take(opb) : nullptr;
+    mref.Gt_q = dith ? (double*)cp.take(rows * sizeof(double)) : nullptr;
+    double* rot = (double*)cp.take(rows * 9 * sizeof(double));
+    meps2 = (float*)cp.take((size_t)n * sizeof(float));
+    LM.thr = (unsigned*)cp.take((size_t)n * sizeof(unsigned));
+    double* gm = (double*)cp.take(2 * sizeof(double));
+    mref.gmax = gm;
+    if (!mref.cen || !mref.G || !mref.Gt || !mref.op || !rot || !meps2 || !LM.thr || !gm ||
+        (dith && (!mref.op_q || !mref.Gt_q)))
+      return fail(c, MDSCTK_ERR_ARG, "symmetric self query: canonical workspace carve");
+    const DevFrames mq = query_view(mref);
+    HIP_TRY(c, launch_prep(mref, c->w, dith ? +1 : 0, dith ? &mq : nullptr, -1, s.st,
+                           c->ref.xyz, order, rot));
+    HIP_TRY(c, launch_max_gt(mref, gm, s.st));
+    mfit = mq;
+    HIP_TRY(c, launch_eps_sym(mfit, mref, kappa, gm + 1, meps2, nullptr, s.st));
+    // the landmark thresholds in the new order, widened by the new window
+    // (the landmark keys came from the original operand, the main pass's
+    // from the rotated one: both within their windows of the exact scores)
+    HIP_TRY(c, launch_permute_thresholds(L.thr, order, meps2, n, LM.thr, s.st));
+  }
+  HIP_TRY(c, launch_init_lists(LM, n, s.st, false));
+  HIP_TRY(c, launch_mirror_init(M, n, s.st));
+  HIP_TRY(c, hipEventRecord(s.ev[6], s.st));
+  HIP_TRY(c, launch_contract(mref, mfit, LM, meps2, q.k1, phases, q.nofit, nullptr, grid, s.st,
+                             &M));
+  HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
+  HIP_TRY(c, launch_mirror_sum(M, n, msum, s.st));
+  HIP_TRY(c, launch_merge(c->ref, fit, c->w, LM, meps2, q.k1, q.k_out, q.rank_off, q.nofit, d_dist,
+                          d_idx, d_fallback, s.n_cand, (ordered && !canon) ? order : nullptr, s.st,
+                          &M, canon ? order : nullptr));
   HIP_TRY(c, hipEventRecord(s.ev[4], s.st));
   // pairs of the upper block triangle: fit block fb against frames from
   // its own first frame on

@@ -131,9 +131,14 @@ struct Mirror {
 // side 0: plain operand; +1 / -1: the dithered split's reference / query
 // side (split forms only).  second (may be null): a view of the same frames
 // whose op / Gt receive the operand and G~ of side2 as well (the resident
-// set's query-side operand).
+// set's query-side operand).  src_xyz (if given): the frames are read from
+// it instead of f.xyz, row r from frame perm[r] when perm is given.  rot (if
+// given, frames only): every frame is expressed in its principal axes,
+// whose rotation is stored at rot[9 r] (frame_stats_kernel).
 hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side,
-                       const DevFrames* second, int side2, hipStream_t st);
+                       const DevFrames* second, int side2, hipStream_t st,
+                       const float* src_xyz = nullptr, const int* perm = nullptr,
+                       double* rot = nullptr);
 // reset_thr false keeps the shared thresholds (a landmark pass set them).
 hipError_t launch_init_lists(const Lists& L, int64_t n_fit, hipStream_t st,
                              bool reset_thr = true);
@@ -178,12 +183,20 @@ hipError_t launch_scatter_rows(const double* sd, const int* si, const int* rows,
                                int k_out, double* dd, int* di, hipStream_t st);
 // order: the query rows in the order the merge workgroups take them
 // (nullptr: 0..n_fit-1).  mirror: a symmetric self query's second list per
-// row (nullptr: none).
+// row (nullptr: none).  perm (replaces order): the lists, mirrored lists and
+// eps2 were built over the frames in the order perm (list row i is frame
+// perm[i], entry index j frame perm[j]); ref/fit and the outputs and
+// fallback flags use the original indices.
 hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
                         const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
                         bool nofit, double* out_dist, int* out_idx, int* fallback,
                         int* n_cand, const int* order, hipStream_t st,
-                        const Mirror* mirror = nullptr);
+                        const Mirror* mirror = nullptr, const int* perm = nullptr);
+// dst[i] = the float bits src[perm[i]] raised by add[i] (and one unit in the
+// last place): the shared thresholds of a landmark pass moved to the
+// permuted order, widened by the permuted frames' window.
+hipError_t launch_permute_thresholds(const unsigned* src, const int* perm, const float* add,
+                                     int64_t n, unsigned* dst, hipStream_t st);
 // Exact FP64 scores s = G_a + G_b - 2 lambda for fit rows [row0,row0+nrows)
 // against all refs.  Writes s into srow[(row-row0)*n_ref + ref] unless
 // out_full is given, in which case it writes Angstrom distances into

@@ -224,6 +224,20 @@ __device__ __forceinline__ double qcp_score_approx(const float* m, double ga, do
   return (s == s && s < INFINITY) ? s : 0.0;
 }
 
+// An upper bound of lambda_max(K(m)), the largest eigenvalue of the key
+// matrix of the 3x3 cross-covariance m: lambda_max = max over rotations R of
+// tr(R m) = sum_j <row j of R, column j of m> <= the sum of the column norms
+// of m, since every row of R has norm 1.  Evaluated in FP32 and raised by
+// 1e-5 relative, which covers the rounding of the norms and their sum (a
+// few units of 2^-24 each).  For two similar frames, each expressed in its
+// principal axes, m is nearly diagonal and the bound nearly exact.
+__device__ __forceinline__ float lambda_upper(const float* m) {
+  float cs = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) cs += sqrtf(fmaf(m[i], m[i], fmaf(m[3 + i], m[3 + i], m[6 + i] * m[6 + i])));
+  return cs * (1.0f + 1e-5f);
+}
+
 // ---------------------------------------------------------------------------
 // prep: one 192-thread workgroup per 32-frame tile.  Thread t -> plane
 // c = t/64, lane l = t%64 of that plane's operand block.
@@ -263,6 +277,82 @@ __device__ __forceinline__ double side_scale(int side, int a) {
   return side == 0 ? 1.0 : side > 0 ? atom_scale(a) : 1.0 / atom_scale(a);
 }
 
+// Principal axes of a frame: the eigenvectors of T = sum w d d^T (d the
+// centred coordinates; T symmetric 3x3, upper triangle t = (xx, xy, xz, yy,
+// yz, zz)) by cyclic Jacobi in FP64, as the columns of a proper rotation Q
+// (row-major q[3*i + j] = component i of axis j), axes by decreasing
+// eigenvalue, the first two signed so that their largest component is
+// positive and the third their cross product.  Every lane of a wave computes
+// the same Q from the same sums.  The RMSD after optimal superposition does
+// not depend on the orientation of either frame, so expressing every frame
+// in its own axes changes no score; it makes the cross-covariance of two
+// similar frames nearly diagonal, which is what the contraction's rejection
+// bound needs.
+__device__ void canonical_axes(const double* t, double* q) {
+  double a[3][3] = {{t[0], t[1], t[2]}, {t[1], t[3], t[4]}, {t[2], t[4], t[5]}};
+  double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    const double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+    const double scale = fabs(a[0][0]) + fabs(a[1][1]) + fabs(a[2][2]);
+    if (off <= 1e-15 * scale || off == 0.0) break;
+    for (int pq = 0; pq < 3; ++pq) {
+      const int p = pq == 2 ? 1 : 0, r = pq == 0 ? 1 : 2;
+      const double apr = a[p][r];
+      if (apr == 0.0) continue;
+      const double th = (a[r][r] - a[p][p]) / (2.0 * apr);
+      double tt = 1.0 / (fabs(th) + sqrt(th * th + 1.0));
+      if (th < 0.0) tt = -tt;
+      const double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
+      for (int k = 0; k < 3; ++k) {
+        const double akp = a[k][p], akr = a[k][r];
+        a[k][p] = cs * akp - sn * akr;
+        a[k][r] = sn * akp + cs * akr;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double apk = a[p][k], ark = a[r][k];
+        a[p][k] = cs * apk - sn * ark;
+        a[r][k] = sn * apk + cs * ark;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double vkp = v[k][p], vkr = v[k][r];
+        v[k][p] = cs * vkp - sn * vkr;
+        v[k][r] = sn * vkp + cs * vkr;
+      }
+    }
+  }
+  // order by decreasing eigenvalue
+  int o[3] = {0, 1, 2};
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2 - i; ++j)
+      if (a[o[j]][o[j]] < a[o[j + 1]][o[j + 1]]) {
+        const int x = o[j];
+        o[j] = o[j + 1];
+        o[j + 1] = x;
+      }
+  double e[2][3];
+  for (int c = 0; c < 2; ++c) {
+    int big = 0;
+    for (int i = 1; i < 3; ++i)
+      if (fabs(v[i][o[c]]) > fabs(v[big][o[c]])) big = i;
+    const double sg = v[big][o[c]] < 0.0 ? -1.0 : 1.0;
+    for (int i = 0; i < 3; ++i) e[c][i] = sg * v[i][o[c]];
+  }
+  const double e2[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                        e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+  for (int i = 0; i < 3; ++i) {
+    q[3 * i + 0] = e[0][i];
+    q[3 * i + 1] = e[1][i];
+    q[3 * i + 2] = e2[i];
+  }
+}
+
+// The centred coordinate e along axis (q0, q1, q2), one rounding order
+// wherever it is formed (G~ and the operand must see the same value).
+__device__ __forceinline__ double axis_coord(double e0, double e1, double e2, double q0, double q1,
+                                             double q2) {
+  return fma(e2, q2, fma(e1, q1, e0 * q0));
+}
+
 // One wave per frame row (rows up to n_tiles*32; padded rows get zeros),
 // reading the frame as one contiguous run of 12-byte atoms:
 //   cen  FP64 mass-weighted centroid (center == 0, data vectors: 0)
@@ -271,18 +361,24 @@ __device__ __forceinline__ double side_scale(int side, int a) {
 //        represents them (scale of the dither side taken back out); Gt2,
 //        when given, the same for a second side (the resident set's
 //        query-side operand).
+// perm: row f holds frame perm[f] of xyz (else frame f).  rot: the frame's
+// coordinates are taken in its principal axes (canonical_axes, Q stored at
+// rot[9 f]), and G~ is that of the rotated operand.
 template <int PREC>
 __global__ __launch_bounds__(256) void frame_stats_kernel(
     const float* __restrict__ xyz, int64_t n, int64_t rows, int natoms,
     const float* __restrict__ w, const float* __restrict__ sw, double Wtot, int center,
     int side, int side2, double* __restrict__ cen, double* __restrict__ G,
-    double* __restrict__ Gt, double* __restrict__ Gt2) {
+    double* __restrict__ Gt, double* __restrict__ Gt2, const int* __restrict__ perm,
+    double* __restrict__ rot) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t f = (int64_t)blockIdx.x * 4 + wave;
   if (f >= rows) return;
   double c0 = 0.0, c1 = 0.0, c2 = 0.0, g = 0.0, gt = 0.0, gt2 = 0.0;
+  double q[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   if (f < n) {
-    const F3* x = reinterpret_cast<const F3*>(xyz) + f * (int64_t)natoms;
+    const int64_t src = perm ? (int64_t)perm[f] : f;
+    const F3* x = reinterpret_cast<const F3*>(xyz) + src * (int64_t)natoms;
     if (center) {
       for (int a = lane; a < natoms; a += 64) {
         const double wa = (double)w[a];
@@ -295,10 +391,27 @@ __global__ __launch_bounds__(256) void frame_stats_kernel(
       c1 = wave_sum_d(c1) / Wtot;
       c2 = wave_sum_d(c2) / Wtot;
     }
+    if (rot) {
+      double t[6] = {0, 0, 0, 0, 0, 0};
+      for (int a = lane; a < natoms; a += 64) {
+        const F3 v = x[a];
+        const double wa = (double)w[a];
+        const double d0 = (double)v.x - c0, d1 = (double)v.y - c1, d2 = (double)v.z - c2;
+        t[0] += wa * d0 * d0; t[1] += wa * d0 * d1; t[2] += wa * d0 * d2;
+        t[3] += wa * d1 * d1; t[4] += wa * d1 * d2; t[5] += wa * d2 * d2;
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i) t[i] = wave_sum_d(t[i]);
+      canonical_axes(t, q);
+    }
     for (int a = lane; a < natoms; a += 64) {
       const F3 v = x[a];
-      const double d0 = (double)v.x - c0, d1 = (double)v.y - c1, d2 = (double)v.z - c2;
-      g += (double)w[a] * (d0 * d0 + d1 * d1 + d2 * d2);
+      const double e0 = (double)v.x - c0, e1 = (double)v.y - c1, e2 = (double)v.z - c2;
+      g += (double)w[a] * (e0 * e0 + e1 * e1 + e2 * e2);
+      // the coordinates in the frame's axes (the identity without rot)
+      const double d0 = rot ? axis_coord(e0, e1, e2, q[0], q[3], q[6]) : e0;
+      const double d1 = rot ? axis_coord(e0, e1, e2, q[1], q[4], q[7]) : e1;
+      const double d2 = rot ? axis_coord(e0, e1, e2, q[2], q[5], q[8]) : e2;
       if (Gt) {
         const double s = (double)sw[a], sc = side_scale(side, a);
         const double r0 = represented<PREC>(s * d0 * sc) / sc;
@@ -326,6 +439,7 @@ __global__ __launch_bounds__(256) void frame_stats_kernel(
     if (Gt) Gt[f] = gt;
     if (Gt2) Gt2[f] = gt2;
   }
+  if (rot && lane < 9) rot[f * 9 + lane] = q[lane];
 }
 
 // The contraction operand in fragment order, from frame_stats_kernel's
@@ -335,12 +449,15 @@ __global__ __launch_bounds__(256) void frame_stats_kernel(
 // represented<PREC>'s.
 constexpr int kGroupsPerStage = 4;
 
+// perm and rot as frame_stats_kernel (rot: the Q it stored for each row).
 template <int PREC>
 __global__ __launch_bounds__(192) void operand_kernel(const float* __restrict__ xyz, int64_t n,
                                                       int natoms, int n_groups,
                                                       const float* __restrict__ sw,
                                                       const double* __restrict__ cen, int side,
-                                                      uint4* __restrict__ op) {
+                                                      uint4* __restrict__ op,
+                                                      const int* __restrict__ perm,
+                                                      const double* __restrict__ rot) {
   constexpr int KG = (PREC == kFP32) ? 8 : 16;
   constexpr int CH = KG * 3;                     // floats of one frame in one K-group
   constexpr int SW = CH * kGroupsPerStage;       // floats of one frame in one stage
@@ -355,14 +472,32 @@ __global__ __launch_bounds__(192) void operand_kernel(const float* __restrict__ 
   const int lane = t & 63;
   const int row = lane & 31;
   const int h = lane >> 5;
-  const double cm = cen[((int64_t)tile * kTile + row) * 3 + c];
+  const int64_t frow = (int64_t)tile * kTile + row;
+  const double cm0 = cen[frow * 3 + 0], cm1 = cen[frow * 3 + 1], cm2 = cen[frow * 3 + 2];
+  // plane c of the operand: column c of the frame's Q (or axis c)
+  double qc0 = c == 0, qc1 = c == 1, qc2 = c == 2;
+  if (rot && frow < n) {
+    qc0 = rot[frow * 9 + 0 + c];
+    qc1 = rot[frow * 9 + 3 + c];
+    qc2 = rot[frow * 9 + 6 + c];
+  }
+  // the centred coordinate of local atom la along plane c
+  auto coord = [&](const float* xr, int la) -> double {
+    const double e0 = (double)xr[la * 3 + 0] - cm0, e1 = (double)xr[la * 3 + 1] - cm1,
+                 e2 = (double)xr[la * 3 + 2] - cm2;
+    if (!rot) return c == 0 ? e0 : c == 1 ? e1 : e2;
+    return axis_coord(e0, e1, e2, qc0, qc1, qc2);
+  };
 
   for (int i = t; i < kTile * SW; i += 192) {
     const int fr = i / SW, e = i - fr * SW;
     const int64_t ff = (int64_t)tile * kTile + fr;
     const int a = g0 * KG + e / 3;
     float v = 0.f;
-    if (ff < n && a < natoms) v = xyz[(ff * natoms + (int64_t)g0 * KG) * 3 + e];
+    if (ff < n && a < natoms) {
+      const int64_t src = perm ? (int64_t)perm[ff] : ff;
+      v = xyz[(src * natoms + (int64_t)g0 * KG) * 3 + e];
+    }
     xs[fr * SP + e] = v;
   }
   __syncthreads();
@@ -377,7 +512,7 @@ __global__ __launch_bounds__(192) void operand_kernel(const float* __restrict__ 
       for (int j = 0; j < 4; ++j) {
         const int la = local_atom(j);
         const int a = g * KG + la;
-        v4[j] = a < natoms ? (float)((double)sw[a] * ((double)xr[la * 3 + c] - cm)) : 0.f;
+        v4[j] = a < natoms ? (float)((double)sw[a] * coord(xr, la)) : 0.f;
       }
       uint4 o;
       o.x = __float_as_uint(v4[0]);
@@ -392,7 +527,7 @@ __global__ __launch_bounds__(192) void operand_kernel(const float* __restrict__ 
         const int la = local_atom(j);
         const int a = g * KG + la;
         double v = 0.0;
-        if (a < natoms) v = (double)sw[a] * ((double)xr[la * 3 + c] - cm) * side_scale(side, a);
+        if (a < natoms) v = (double)sw[a] * coord(xr, la) * side_scale(side, a);
         if constexpr (PREC == kBF16) {
           hi[j] = bf16_rne((float)v);
           lo[j] = bf16_rne((float)(v - bf16_val(hi[j])));
@@ -1021,10 +1156,21 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
         bool own = false, mir = false;
         float key = 0.f;
         if (valid) {
-          const double s = qcp_score_approx(m, gr, gf, fmax((double)thr, (double)tr), P.nofit);
-          key = (float)fmax(s, 0.0);
-          own = s <= (double)thr;
-          mir = s <= (double)tr;
+          const double rej = fmax((double)thr, (double)tr);
+          // A pair whose score is above every list it could join, by the
+          // bound on lambda_max, skips the QCP: the QCP would reject it too.
+          // Its Newton iterate approaches the score from below, to within
+          // far less than 1e-9 (G_r + G_f) (the tolerance of its last step and
+          // its rounding), which the test allows for.  (The trace score of
+          // --nofit has no QCP to skip.)
+          const bool far =
+              !P.nofit && gr + gf - 2.0 * (double)lambda_upper(m) - 1e-9 * (gr + gf) > rej;
+          if (!far) {
+            const double s = qcp_score_approx(m, gr, gf, rej, P.nofit);
+            key = (float)fmax(s, 0.0);
+            own = s <= (double)thr;
+            mir = s <= (double)tr;
+          }
         }
         if (own) {
           // The compaction keeps cnt <= cap - 16 between tiles, so the guard
@@ -1190,6 +1336,11 @@ __device__ __forceinline__ double wave_data_pair_score(const PairSource& P, int6
 struct MergeParams {
   PairSource src;
   const int* order;  // query rows in processing order (nullptr: 0..n_fit-1)
+  // the lists were built over a permutation of the frames (the symmetric
+  // self query in landmark order): list row i and list entry index j are
+  // frames perm[i] and perm[j]; the frames' data and the outputs use the
+  // original indices.  Replaces order (the lists are in processing order).
+  const int* perm;
   double W, out_scale;
   const uint2* ent;
   const int* cnt;
@@ -1244,7 +1395,8 @@ constexpr int kMergeStageAtoms = 1024;
 // waves-per-EU hint keeps it within 96 VGPRs (5 workgroups per CU).
 template <bool STAGE, int CAND>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void merge_kernel(MergeParams P) {
-  const int64_t q = P.order ? P.order[blockIdx.x] : blockIdx.x;
+  const int64_t qi = P.perm ? (int64_t)blockIdx.x : (P.order ? P.order[blockIdx.x] : blockIdx.x);
+  const int64_t q = P.perm ? P.perm[qi] : qi;  // the frame: data and output row
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   __shared__ unsigned hist[256];
   __shared__ unsigned s_prefix;
@@ -1259,11 +1411,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
   __shared__ double guard_s[kMergeGuard];
   __shared__ int s_ng;
 
-  const int* cnt = P.cnt + q * P.nl;
-  const uint2* ent = P.ent + q * P.nl * (int64_t)P.cap;
-  const float* ovf = P.ovf + q * P.nl;
-  const int mc = P.ment ? min(P.mcnt[q], P.mcap) : 0;
-  const uint2* ment = P.ment ? P.ment + q * (int64_t)P.mcap : nullptr;
+  const int* cnt = P.cnt + qi * P.nl;
+  const uint2* ent = P.ent + qi * P.nl * (int64_t)P.cap;
+  const float* ovf = P.ovf + qi * P.nl;
+  const int mc = P.ment ? min(P.mcnt[qi], P.mcap) : 0;
+  const uint2* ment = P.ment ? P.ment + qi * (int64_t)P.mcap : nullptr;
+  auto frame_of = [&](unsigned j) -> int { return P.perm ? P.perm[j] : (int)j; };
   // f(entry) for every entry of the row's lists (waves over the lists,
   // lanes over entries; the whole workgroup over the mirrored list)
   auto visit = [&](auto&& f) {
@@ -1313,26 +1466,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
     kk = s_kk;
   }
   const float tau = __uint_as_float(prefix);
-  const float cut = tau * (1.0f + 2.4e-7f) + P.eps2[q];
+  const float e2q = P.eps2[qi];
+  const float cut = tau * (1.0f + 2.4e-7f) + e2q;
   // The guard range: one more window width above the cut.  A window too
   // narrow for its row would drop pairs whose keys lie just above the cut,
   // so up to kMergeGuard of those are re-scored as well.
-  const float guard_cut = cut + P.eps2[q];
+  const float guard_cut = cut + e2q;
   for (int l = tid; l < P.nl; l += 256)
     if (ovf[l] <= cut) s_flag = 1;
-  if (tid == 0 && P.ment && P.movf[q] <= cut) s_flag = 1;
+  if (tid == 0 && P.ment && P.movf[qi] <= cut) s_flag = 1;
   visit([&](const uint2 e) {
     const float key = __uint_as_float(e.x);
     if (key <= cut) {
       const int pos = atomicAdd(&s_nc, 1);
       if (pos < CAND) {
-        cand_idx[pos] = (int)e.y;
+        cand_idx[pos] = frame_of(e.y);
         cand_key[pos] = key;
       }
     } else if (key <= guard_cut) {
       const int pos = atomicAdd(&s_ng, 1);
       if (pos < kMergeGuard) {
-        guard_idx[pos] = (int)e.y;
+        guard_idx[pos] = frame_of(e.y);
         guard_key[pos] = key;
       }
     }
@@ -1392,7 +1546,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
   // the exact path.  The keys are rounded to FP32, hence the relative
   // allowance.
   {
-    const double eps = 0.5 * (double)P.eps2[q];
+    const double eps = 0.5 * (double)e2q;
     for (int c = tid; c < nc + ng; c += 256) {
       const double s = c < nc ? cand_s[c] : guard_s[c - nc];
       const double kq = (double)(c < nc ? cand_key[c] : guard_key[c - nc]);
@@ -1599,6 +1753,17 @@ __global__ void mirror_sum_kernel(const int* __restrict__ mcnt, int64_t n, int m
 
 // Rows of a frame set gathered into a batch (rows[i] -> dst row i), and
 // output rows scattered back (src row i -> dst row rows[i]).
+__global__ void permute_thresholds_kernel(const unsigned* __restrict__ src,
+                                          const int* __restrict__ perm,
+                                          const float* __restrict__ add, int64_t n,
+                                          unsigned* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float t = __uint_as_float(src[perm[i]]);
+  const float v = t + add[i];
+  dst[i] = v < INFINITY ? __float_as_uint(v) + 1u : kInfBits;
+}
+
 __global__ void gather_frames_kernel(const float* __restrict__ src, const int* __restrict__ rows,
                                      int natoms, float* __restrict__ dst) {
   const int64_t i = blockIdx.x;
@@ -1974,8 +2139,11 @@ int list_capacity_for(int k1) {
 }
 
 hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side,
-                       const DevFrames* second, int side2, hipStream_t st) {
+                       const DevFrames* second, int side2, hipStream_t st, const float* src_xyz,
+                       const int* perm, double* rot) {
   const int64_t rows = (int64_t)f.n_tiles * kTile;
+  const float* xyz = src_xyz ? src_xyz : f.xyz;
+  if (rot && w.metric != kMetricRMSD) return hipErrorInvalidValue;
   const int center = w.metric == kMetricRMSD ? 1 : 0;
   double* gt = f.op ? f.Gt : nullptr;
   double* gt2 = (f.op && second) ? second->Gt : nullptr;
@@ -1983,14 +2151,15 @@ hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side,
   const dim3 ogrid((unsigned)f.n_tiles, (unsigned)((f.n_groups + kGroupsPerStage - 1) / kGroupsPerStage));
 #define MDSCTK_PREP(PR)                                                                       \
   do {                                                                                        \
-    hipLaunchKernelGGL(frame_stats_kernel<PR>, sgrid, sblock, 0, st, f.xyz, f.n, rows,        \
-                       f.natoms, w.w, w.sw, w.W, center, side, side2, f.cen, f.G, gt, gt2);   \
+    hipLaunchKernelGGL(frame_stats_kernel<PR>, sgrid, sblock, 0, st, xyz, f.n, rows,          \
+                       f.natoms, w.w, w.sw, w.W, center, side, side2, f.cen, f.G, gt, gt2,    \
+                       perm, rot);                                                            \
     if (f.op)                                                                                 \
-      hipLaunchKernelGGL(operand_kernel<PR>, ogrid, dim3(192), 0, st, f.xyz, f.n, f.natoms,   \
-                         f.n_groups, w.sw, f.cen, side, f.op);                               \
+      hipLaunchKernelGGL(operand_kernel<PR>, ogrid, dim3(192), 0, st, xyz, f.n, f.natoms,     \
+                         f.n_groups, w.sw, f.cen, side, f.op, perm, rot);                    \
     if (f.op && second)                                                                       \
-      hipLaunchKernelGGL(operand_kernel<PR>, ogrid, dim3(192), 0, st, f.xyz, f.n, f.natoms,   \
-                         f.n_groups, w.sw, f.cen, side2, second->op);                        \
+      hipLaunchKernelGGL(operand_kernel<PR>, ogrid, dim3(192), 0, st, xyz, f.n, f.natoms,     \
+                         f.n_groups, w.sw, f.cen, side2, second->op, perm, rot);             \
   } while (0)
   switch (f.prec) {
     case kFP32: MDSCTK_PREP(kFP32); break;
@@ -2141,7 +2310,7 @@ static PairSource pair_source(const DevFrames& ref, const DevFrames& fit, const 
 hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWeights& w,
                         const Lists& L, const float* eps2, int k1, int k_out, int rank_off,
                         bool nofit, double* out_dist, int* out_idx, int* fallback, int* n_cand,
-                        const int* order, hipStream_t st, const Mirror* mirror) {
+                        const int* order, hipStream_t st, const Mirror* mirror, const int* perm) {
   MergeParams P;
   P.src = pair_source(ref, fit, w);
   P.W = w.W;
@@ -2165,6 +2334,7 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
   P.fallback = fallback;
   P.n_cand = n_cand;
   P.order = order;
+  P.perm = perm;
   // k1 <= 256: windows of up to 1024 candidates (C3: 184 per query), half
   // the candidate LDS of the general instance
   const bool stage = w.metric == kMetricRMSD && ref.natoms <= kMergeStageAtoms;
@@ -2181,6 +2351,14 @@ hipError_t launch_merge(const DevFrames& ref, const DevFrames& fit, const DevWei
     else
       hipLaunchKernelGGL((merge_kernel<false, kCandMax>), grid, block, dyn, st, P);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_permute_thresholds(const unsigned* src, const int* perm, const float* add,
+                                     int64_t n, unsigned* dst, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(permute_thresholds_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     src, perm, add, n, dst);
   return hipGetLastError();
 }
 
