@@ -379,29 +379,31 @@ __global__ __launch_bounds__(256) void frame_stats_kernel(
   if (f < n) {
     const int64_t src = perm ? (int64_t)perm[f] : f;
     const F3* x = reinterpret_cast<const F3*>(xyz) + src * (int64_t)natoms;
+    // with rot, the second moments in the same pass: T = sum w x x^T - W c c^T
+    // (its cancellation only perturbs the axes, which need not be exact)
+    double t[6] = {0, 0, 0, 0, 0, 0};
     if (center) {
       for (int a = lane; a < natoms; a += 64) {
         const double wa = (double)w[a];
         const F3 v = x[a];
-        c0 += wa * (double)v.x;
-        c1 += wa * (double)v.y;
-        c2 += wa * (double)v.z;
+        const double x0 = (double)v.x, x1 = (double)v.y, x2 = (double)v.z;
+        c0 += wa * x0;
+        c1 += wa * x1;
+        c2 += wa * x2;
+        if (rot) {
+          t[0] += wa * x0 * x0; t[1] += wa * x0 * x1; t[2] += wa * x0 * x2;
+          t[3] += wa * x1 * x1; t[4] += wa * x1 * x2; t[5] += wa * x2 * x2;
+        }
       }
       c0 = wave_sum_d(c0) / Wtot;
       c1 = wave_sum_d(c1) / Wtot;
       c2 = wave_sum_d(c2) / Wtot;
     }
     if (rot) {
-      double t[6] = {0, 0, 0, 0, 0, 0};
-      for (int a = lane; a < natoms; a += 64) {
-        const F3 v = x[a];
-        const double wa = (double)w[a];
-        const double d0 = (double)v.x - c0, d1 = (double)v.y - c1, d2 = (double)v.z - c2;
-        t[0] += wa * d0 * d0; t[1] += wa * d0 * d1; t[2] += wa * d0 * d2;
-        t[3] += wa * d1 * d1; t[4] += wa * d1 * d2; t[5] += wa * d2 * d2;
-      }
 #pragma unroll
       for (int i = 0; i < 6; ++i) t[i] = wave_sum_d(t[i]);
+      t[0] -= Wtot * c0 * c0; t[1] -= Wtot * c0 * c1; t[2] -= Wtot * c0 * c2;
+      t[3] -= Wtot * c1 * c1; t[4] -= Wtot * c1 * c2; t[5] -= Wtot * c2 * c2;
       canonical_axes(t, q);
     }
     for (int a = lane; a < natoms; a += 64) {
@@ -450,6 +452,8 @@ __global__ __launch_bounds__(256) void frame_stats_kernel(
 constexpr int kGroupsPerStage = 4;
 
 // perm and rot as frame_stats_kernel (rot: the Q it stored for each row).
+// op2 (split forms): the operand of a second dither side, side2, from the
+// same staged frames.
 template <int PREC>
 __global__ __launch_bounds__(192) void operand_kernel(const float* __restrict__ xyz, int64_t n,
                                                       int natoms, int n_groups,
@@ -457,7 +461,8 @@ __global__ __launch_bounds__(192) void operand_kernel(const float* __restrict__ 
                                                       const double* __restrict__ cen, int side,
                                                       uint4* __restrict__ op,
                                                       const int* __restrict__ perm,
-                                                      const double* __restrict__ rot) {
+                                                      const double* __restrict__ rot, int side2,
+                                                      uint4* __restrict__ op2) {
   constexpr int KG = (PREC == kFP32) ? 8 : 16;
   constexpr int CH = KG * 3;                     // floats of one frame in one K-group
   constexpr int SW = CH * kGroupsPerStage;       // floats of one frame in one stage
@@ -520,30 +525,40 @@ __global__ __launch_bounds__(192) void operand_kernel(const float* __restrict__ 
       o.z = __float_as_uint(v4[2]);
       o.w = __float_as_uint(v4[3]);
       op[(((int64_t)tile * n_groups + g) * 3 + c) * 64 + lane] = o;
+      if (op2) op2[(((int64_t)tile * n_groups + g) * 3 + c) * 64 + lane] = o;  // no dither sides
     } else {
-      unsigned short hi[8], lo[8];
+      double base[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int la = local_atom(j);
         const int a = g * KG + la;
-        double v = 0.0;
-        if (a < natoms) v = (double)sw[a] * coord(xr, la) * side_scale(side, a);
-        if constexpr (PREC == kBF16) {
-          hi[j] = bf16_rne((float)v);
-          lo[j] = bf16_rne((float)(v - bf16_val(hi[j])));
-        } else {
-          hi[j] = f16_rne((float)v);
-          lo[j] = f16_rne((float)(v - f16_val(hi[j])));
-        }
+        base[j] = a < natoms ? (double)sw[a] * coord(xr, la) : 0.0;
       }
-      uint4 oh, ol;
-      oh.x = hi[0] | ((unsigned)hi[1] << 16); oh.y = hi[2] | ((unsigned)hi[3] << 16);
-      oh.z = hi[4] | ((unsigned)hi[5] << 16); oh.w = hi[6] | ((unsigned)hi[7] << 16);
-      ol.x = lo[0] | ((unsigned)lo[1] << 16); ol.y = lo[2] | ((unsigned)lo[3] << 16);
-      ol.z = lo[4] | ((unsigned)lo[5] << 16); ol.w = lo[6] | ((unsigned)lo[7] << 16);
       const int64_t blk = (((int64_t)tile * n_groups + g) * 6 + 2 * c);
-      op[blk * 64 + lane] = oh;
-      op[(blk + 1) * 64 + lane] = ol;
+      auto emit = [&](int sd, uint4* __restrict__ dst) {
+        unsigned short hi[8], lo[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int a = g * KG + local_atom(j);
+          const double v = a < natoms ? base[j] * side_scale(sd, a) : 0.0;
+          if constexpr (PREC == kBF16) {
+            hi[j] = bf16_rne((float)v);
+            lo[j] = bf16_rne((float)(v - bf16_val(hi[j])));
+          } else {
+            hi[j] = f16_rne((float)v);
+            lo[j] = f16_rne((float)(v - f16_val(hi[j])));
+          }
+        }
+        uint4 oh, ol;
+        oh.x = hi[0] | ((unsigned)hi[1] << 16); oh.y = hi[2] | ((unsigned)hi[3] << 16);
+        oh.z = hi[4] | ((unsigned)hi[5] << 16); oh.w = hi[6] | ((unsigned)hi[7] << 16);
+        ol.x = lo[0] | ((unsigned)lo[1] << 16); ol.y = lo[2] | ((unsigned)lo[3] << 16);
+        ol.z = lo[4] | ((unsigned)lo[5] << 16); ol.w = lo[6] | ((unsigned)lo[7] << 16);
+        dst[blk * 64 + lane] = oh;
+        dst[(blk + 1) * 64 + lane] = ol;
+      };
+      emit(side, op);
+      if (op2) emit(side2, op2);
     }
   }
 }
@@ -2156,10 +2171,8 @@ hipError_t launch_prep(const DevFrames& f, const DevWeights& w, int side,
                        perm, rot);                                                            \
     if (f.op)                                                                                 \
       hipLaunchKernelGGL(operand_kernel<PR>, ogrid, dim3(192), 0, st, xyz, f.n, f.natoms,     \
-                         f.n_groups, w.sw, f.cen, side, f.op, perm, rot);                    \
-    if (f.op && second)                                                                       \
-      hipLaunchKernelGGL(operand_kernel<PR>, ogrid, dim3(192), 0, st, xyz, f.n, f.natoms,     \
-                         f.n_groups, w.sw, f.cen, side2, second->op, perm, rot);             \
+                         f.n_groups, w.sw, f.cen, side, f.op, perm, rot, side2,              \
+                         second ? second->op : nullptr);                                      \
   } while (0)
   switch (f.prec) {
     case kFP32: MDSCTK_PREP(kFP32); break;
