@@ -1,9 +1,12 @@
 """GPU parity of the symmetric self query: a landmark pass bounds every
 row's threshold, the contraction scores the upper block triangle once into
 each fit frame's candidate lists and each reference frame's mirrored list
-(no score matrix), and the merge re-scores both.  Its outputs must equal the
-candidate-list path's bit for bit (both re-score a certified window in
-FP64), and the FP64 oracle's within the distance tolerance.
+(no score matrix), and the merge re-scores both.  From 512 frames on (fitted
+RMSD) the contraction runs over a copy of the set in landmark order, each
+frame in its own principal axes, and skips the QCP of pairs that a bound on
+lambda_max rejects.  Its outputs must equal the candidate-list path's bit
+for bit (both re-score a certified window in FP64), and the FP64 oracle's
+within the distance tolerance.
 
 All calls go through the C ABI (libmdsctk_knn.so via ctypes)."""
 import numpy as np
@@ -62,8 +65,11 @@ def test_symmetric_equals_lists(prec, n):
 @pytest.mark.parametrize("prec", ["bf16", "fp32"])
 @pytest.mark.parametrize("flags", [knn.KNN_NOFIT, knn.KNN_KEEP_RANK0,
                                    knn.KNN_NOFIT | knn.KNN_KEEP_RANK0])
-def test_symmetric_flags(prec, flags):
-    frames = synth.trajectory(450, 57, 5)
+@pytest.mark.parametrize("n", [450, 700])
+def test_symmetric_flags(prec, flags, n):
+    """450 frames: the set in its own order; 700: the landmark-ordered copy
+    in principal axes (fitted scores only)."""
+    frames = synth.trajectory(n, 57, 5)
     w = synth.masses(57, "random", 5)
     _both(prec, frames, w, 9, flags)
 
