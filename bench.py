@@ -449,10 +449,17 @@ def measure_traffic(args):
     w_scale = COPY_PROBE_BYTES / (max(copy_w) * 1024.0)
     fetch_b = max(con_f) * 1024.0 * f_scale
     write_b = max(con_w) * 1024.0 * w_scale
-    return fetch_b + write_b, {"fetch_bytes": fetch_b, "write_bytes": write_b,
-                               "fetch_scale": f_scale, "write_scale": w_scale,
-                               "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, KiB per dispatch, "
-                                         "scaled by a 1 GiB hipMemcpy device-to-device copy"}
+    info = {"fetch_bytes": fetch_b, "write_bytes": write_b,
+            "fetch_scale": f_scale, "write_scale": w_scale,
+            "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, KiB per dispatch, "
+                      "scaled by a 1 GiB hipMemcpy device-to-device copy"}
+    # the same passes' merge_kernel dispatch (the query's second kernel)
+    mer_f = [v for n, v in r["fetch"].values() if "merge_kernel" in n]
+    mer_w = [v for n, v in r["write"].values() if "merge_kernel" in n]
+    if mer_f and mer_w:
+        info["merge_kernel"] = {"fetch_bytes": max(mer_f) * 1024.0 * f_scale,
+                                "write_bytes": max(mer_w) * 1024.0 * w_scale}
+    return fetch_b + write_b, info
 
 
 def traffic_child(args):
@@ -602,6 +609,19 @@ def main():
         out["roofline"]["traffic_detail"] = traffic_info
     out["stages_ms"] = {s: mean_stat(stats, "ms_" + s) for s in ("prep", "contract", "merge", "fallback", "total")}
     out["candidates_per_query"] = mean_stat(stats, "n_candidates") / n
+    # the merge: HBM-bound, each re-scored candidate reads its reference
+    # frame (12 N bytes of FP32 coordinates; the guard keys above the cut,
+    # up to 32 per row, are re-scored too and not counted here)
+    ms_merge = mean_stat(stats, "ms_merge")
+    mbytes = mean_stat(stats, "n_candidates") * 12.0 * natoms
+    m_traffic = None
+    if traffic_info and "merge_kernel" in traffic_info:
+        m_traffic = traffic_info["merge_kernel"]["fetch_bytes"] + traffic_info["merge_kernel"]["write_bytes"]
+    out["merge_roofline"] = {"bound": "hbm", "kernel": "merge_kernel",
+                             "achieved": mbytes / (ms_merge * 1e-3) / 1e9, "peak": 8000.0,
+                             "unit": "GB/s", "frac": mbytes / (ms_merge * 1e-3) / 8e12,
+                             "traffic": m_traffic, "bytes_per_launch": mbytes,
+                             "ms_per_launch": ms_merge}
     out["fallback_rows"] = int(max(s["n_fallback_rows"] for s in stats))
     if computed < pairs:
         # the symmetric self query's landmark pass and mirrored lists
