@@ -168,6 +168,8 @@ def run_serving(knn, synth, oracle, prec, args):
         ctx.set_option(knn.OPT_SERVE_GRAPHS, args.serve_graphs)
         ctx.set_option(knn.OPT_SERVE_SHARE_CUS, args.serve_share_cus)
         ctx.set_option(knn.OPT_SERVE_SLOTS, args.serve_slots)
+        if getattr(args, "serve_max_frames", 0):
+            ctx.set_option(knn.OPT_SERVE_MAX_FRAMES, args.serve_max_frames)
         ctx.set_reference(res, w)
         for nw in (4096, 1, 700, 4096):  # warm-up: sizes the batch slots
             ctx.wait(ctx.submit(pool[:nw], kq))
@@ -501,6 +503,8 @@ def main():
                     help="length of the Poisson serving run (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="mean request arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=2, help="batches in flight when serving")
+    ap.add_argument("--serve-max-frames", type=int, default=0,
+                    help="query frames per served batch (0 = the library default)")
     ap.add_argument("--serve-share-cus", type=int, default=1,
                     help="each served batch's contraction takes 1/slots of the CUs (1) or all (0)")
     ap.add_argument("--serve-graphs", type=int, default=1,
