@@ -81,7 +81,7 @@ extern "C" {
                                      pair outside them breaks the bound unseen */
 #define MDSCTK_OPT_REF_SPLITS 3   /* force the number of reference phases (0 = automatic) */
 #define MDSCTK_OPT_SERVE_SLOTS 4  /* batches in flight at once when serving (1..4, default 2) */
-#define MDSCTK_OPT_SERVE_MAX_FRAMES 5 /* query frames per served batch (default 4096) */
+#define MDSCTK_OPT_SERVE_MAX_FRAMES 5 /* query frames per served batch (default 2048) */
 #define MDSCTK_OPT_SERVE_GRAPHS 6   /* 1 (default): each served batch's kernels are captured
                                        into a hipGraph and replayed; 0: launched directly */
 #define MDSCTK_OPT_SERVE_SHARE_CUS 7 /* 1 (default): each served batch's persistent

@@ -117,7 +117,7 @@ struct mdsctk_knn_ctx {
   double eps_scale = 1.0;
   int forced_phases = 0;
   int serve_slots = 2;
-  int64_t serve_max_frames = 4096;
+  int64_t serve_max_frames = 2048;
   bool serve_graphs = true;  // replay each served batch's kernels as a hipGraph
   bool serve_share_cus = true;  // each batch's contraction takes 1/serve_slots of the CUs
   bool self_sym = true;  // self queries score each unordered pair once when the lists fit
