@@ -54,9 +54,11 @@ def _both(prec, frames, w, k, flags=0):
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32"])
-@pytest.mark.parametrize("n", [2000, 777, 130, 67])
+@pytest.mark.parametrize("n", [2000, 777, 512, 511, 130, 67])
 def test_symmetric_equals_lists(prec, n):
-    """Ragged sizes: n not a multiple of 4, 32, 64 or the fit block."""
+    """Ragged sizes: n not a multiple of 4, 32, 64 or the fit block; 512 and
+    511 frames on either side of the landmark-ordered copy in principal
+    axes."""
     frames = synth.metastable_trajectory(n, 304, 7 + n, n_centres=6, dup_fraction=0.1)
     w = synth.masses(304, "backbone")
     _both(prec, frames, w, min(30, n - 1))
@@ -124,6 +126,18 @@ def test_symmetric_large_set_with_duplicates():
     assert st["n_fallback_rows"] >= 6000
     assert np.abs(d[34000:]).max() < 1e-5
     assert st["n_mirrored_max"] <= 8192
+
+
+@pytest.mark.parametrize("natoms", [1, 2, 3])
+def test_symmetric_degenerate_frames(natoms):
+    """Frames of 1, 2 and 3 atoms (a point, a line, a plane: second-moment
+    matrices of rank 0, 1 and 2, whose principal axes are not unique) in a
+    set large enough for the landmark-ordered copy in principal axes."""
+    frames = synth.trajectory(700, natoms, 41)
+    w = synth.masses(natoms, "random", 3)
+    d, i, _ = _both("bf16", frames, w, 12)
+    d_o, i_o = oracle.knn_exact(frames, frames[:16], w, 12)
+    check_knn(frames, frames[:16], w, d[:16], i[:16], d_o, i_o)
 
 
 def test_symmetric_large_k():
