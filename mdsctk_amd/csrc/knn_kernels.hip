@@ -791,7 +791,10 @@ __device__ __forceinline__ unsigned wave_select(const unsigned (&key)[PER], int 
   return low ? (t | ((1u << low) - 1u)) : t;
 }
 
-// Compact the lists of every lane whose list is nearly full.  The whole wave
+// Compact the lists of every lane whose list is nearly full, or holds 2 k1
+// entries while the lane has no threshold yet (a query without a bound from
+// elsewhere: the first compaction sets one early, so the row's other lists
+// stop taking every pair).  The whole wave
 // takes each such list in turn: it finds (an upper bound, within 2^-7
 // relative, of) the list's k1-th smallest key tau, keeps the entries
 // <= tau + 2 eps -- at most cap-64 of them; when more qualify, only those
@@ -804,7 +807,7 @@ __device__ __attribute__((noinline)) void compact_lists(uint2* __restrict__ ent,
                                               float& thr, float& ovf, float eps2, int k1,
                                               int cap, bool active) {
   const int lane = threadIdx.x & 63;
-  uint64_t need = __ballot(active && cnt > cap - 16);
+  uint64_t need = __ballot(active && (cnt > cap - 16 || (cnt >= 2 * k1 && thr == INFINITY)));
   while (need) {
     const int L = __builtin_ctzll(need);
     need &= need - 1;
