@@ -1435,12 +1435,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void m
   const int mc = P.ment ? min(P.mcnt[qi], P.mcap) : 0;
   const uint2* ment = P.ment ? P.ment + qi * (int64_t)P.mcap : nullptr;
   auto frame_of = [&](unsigned j) -> int { return P.perm ? P.perm[j] : (int)j; };
-  // f(entry) for every entry of the row's lists (waves over the lists,
-  // lanes over entries; the whole workgroup over the mirrored list)
+  // f(entry) for every entry of the row's lists, and the whole workgroup
+  // over the mirrored list.  Few lists: waves over the lists, lanes over
+  // entries (coalesced).  Many lists (a small query batch spread over many
+  // reference phases): a thread per list, streaming its entries with four
+  // loads in flight -- a wave per list would wait on one list's count and
+  // entries at a time, hundreds of times per pass.
+  const bool list_per_thread = P.nl >= 256;
   auto visit = [&](auto&& f) {
-    for (int l = wave; l < P.nl; l += 4) {
-      const int c = cnt[l];
-      for (int i = lane; i < c; i += 64) f(ent[(int64_t)l * P.cap + i]);
+    if (list_per_thread) {
+      for (int l = tid; l < P.nl; l += 256) {
+        const int c = cnt[l];
+        const uint2* el = ent + (int64_t)l * P.cap;
+        int i = 0;
+        for (; i + 4 <= c; i += 4) {
+          const uint2 x0 = el[i], x1 = el[i + 1], x2 = el[i + 2], x3 = el[i + 3];
+          f(x0);
+          f(x1);
+          f(x2);
+          f(x3);
+        }
+        for (; i < c; ++i) f(el[i]);
+      }
+    } else {
+      for (int l = wave; l < P.nl; l += 4) {
+        const int c = cnt[l];
+        for (int i = lane; i < c; i += 64) f(ent[(int64_t)l * P.cap + i]);
+      }
     }
     for (int i = tid; i < mc; i += 256) f(ment[i]);
   };
