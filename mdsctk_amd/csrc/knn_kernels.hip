@@ -190,10 +190,17 @@ __device__ double qcp_lambda_exact(const double* S_in, double ga, double gb) {
   return l * g;
 }
 
-// Approximate score for candidate selection.  Newton from the upper bound
-// l = 1 (i.e. (G_a+G_b)/2) decreases monotonically to lambda_max, so every
-// iterate gives a lower bound of s~; once that bound exceeds `reject` the
-// pair cannot be a candidate and iteration stops.
+// Approximate score for candidate selection.  Newton from an upper bound of
+// lambda_max decreases monotonically to it (the key matrix is symmetric, so
+// the quartic's roots are real and it is convex and increasing right of the
+// largest), so every iterate gives a lower bound of s~; once that bound
+// exceeds `reject` the pair cannot be a candidate and iteration stops.  The
+// start is l = 1, i.e. (G_a+G_b)/2, which bounds lambda_max for exact
+// sums; approximate m can exceed it (then s~ < 0: large G terms against
+// small scores), so the start is raised to the column-norm bound of m when
+// that is larger -- starting below the largest root, an iterate could pass
+// `reject` on its way and drop the pair.
+__device__ __forceinline__ float lambda_upper(const float* m);
 __device__ __forceinline__ double qcp_score_approx(const float* m, double ga, double gb,
                                                    double reject, bool nofit) {
   if (nofit) return ga + gb - 2.0 * ((double)m[0] + (double)m[4] + (double)m[8]);
@@ -208,7 +215,7 @@ __device__ __forceinline__ double qcp_score_approx(const float* m, double ga, do
   double c2, c1, c0;
   qcp_coeffs(S, c2, c1, c0);
   const double rej = 0.5 * reject * ig;
-  double l = 1.0;
+  double l = fmax(1.0, (double)lambda_upper(m) * ig);
   for (int it = 0; it < 24; ++it) {
     const double l2 = l * l;
     const double p = (l2 + c2) * l2 + c1 * l + c0;
@@ -329,6 +336,18 @@ __device__ void canonical_axes(const double* t, double* q) {
         o[j] = o[j + 1];
         o[j + 1] = x;
       }
+  // A frame that is a line to within far better than FP32 resolves
+  // (second moment under 1e-4 of the first) keeps its own orientation: in
+  // their axes such frames align along one axis, and the contraction's
+  // approximate scores of aligned, nearly collinear frames of 1000 atoms and
+  // more were measured at up to 66 times the window (as given: under half
+  // of it; every other generator in its axes: under 0.6).  Any proper
+  // rotation per frame leaves the scores unchanged.
+  if (!(a[o[1]][o[1]] >= 1e-4 * a[o[0]][o[0]])) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) q[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    return;
+  }
   double e[2][3];
   for (int c = 0; c < 2; ++c) {
     int big = 0;
@@ -1168,7 +1187,11 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
           continue;
         }
         // Newton stops once the pair is out of every list it could join.
-        float tr = -1.0f;  // the reference frame's threshold (mirrored scores only)
+        // tr: the reference frame's threshold, for a tile whose scores are
+        // mirrored; otherwise a value below every threshold, which only
+        // feeds the rejection bound (an approximate score can be negative,
+        // so the mirrored append tests `mirror` itself).
+        float tr = -1.0f;
         if constexpr (MODE == kOutSym)
           if (mirror) tr = sThr[t & 1][ri * kTile + rl];
         bool own = false, mir = false;
@@ -1187,7 +1210,7 @@ __global__ __launch_bounds__(64 * kRefTilesPerWG * FT, 2) void contract_kernel(C
             const double s = qcp_score_approx(m, gr, gf, rej, P.nofit);
             key = (float)fmax(s, 0.0);
             own = s <= (double)thr;
-            mir = s <= (double)tr;
+            mir = mirror && s <= (double)tr;
           }
         }
         if (own) {

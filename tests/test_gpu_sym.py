@@ -140,6 +140,31 @@ def test_symmetric_degenerate_frames(natoms):
     check_knn(frames, frames[:16], w, d[:16], i[:16], d_o, i_o)
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32"])
+@pytest.mark.parametrize("gen", ["extended-2000", "extended-2000-wide"])
+@pytest.mark.parametrize("n", [640, 500])
+def test_symmetric_hard_frames(prec, gen, n):
+    """Nearly collinear 2000-atom chains, uniform or 1-200 masses: G terms
+    of about 1e8 nm^2 against scores of about 30, so the approximate scores
+    scatter hundreds below zero (the window covers every pair).  A tile whose
+    scores are not mirrored must not append any of them to a mirrored list
+    (a negative score once passed a sentinel threshold there, and diagonal
+    pairs were then counted twice).  640 frames take the landmark-ordered
+    copy in principal axes, 500 the set in its own order.  Outputs must
+    equal the list path's and the oracle's."""
+    natoms = 2000
+    frames = synth.extended_chain(n, natoms, 23)
+    w = synth.masses(natoms, "wide", 5) if gen.endswith("wide") else synth.masses(natoms, "uniform")
+    d, i, st = _both(prec, frames, w, 10)
+    # mirrored appends come only from pairs whose reference frame lies past
+    # the fit block
+    blk = 64 if prec == "fp32" else 128
+    mirrorable = sum(min(blk, n - f0) * max(0, n - f0 - blk) for f0 in range(0, n, blk))
+    assert st["n_mirrored"] <= mirrorable
+    d_o, i_o = oracle.knn_exact(frames, frames[:12], w, 10)
+    check_knn(frames, frames[:12], w, d[:12], i[:12], d_o, i_o)
+
+
 def test_symmetric_large_k():
     frames = synth.trajectory(1500, 64, 9)
     w = synth.masses(64, "uniform")
