@@ -69,6 +69,29 @@ def test_approx_scores_within_window(prec, gen, natoms, nofit):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_cross_query_on_straight_chains(prec):
+    """Straight 2000-atom chains: G terms of about 1e8 nm^2 against scores of
+    about 30, so approximate sums can exceed (G_a+G_b)/2 and scores go
+    negative.  The candidate lists compact at 2(k+1) entries and set row
+    thresholds, and the QCP's early exit is then live: it must not drop a
+    pair on the way (Newton starts above the largest root).  The window
+    covers every pair here, so 900 references keep the rows within the
+    merge's 1024 candidates (the fused path, not the exact rows).  Rows
+    against the FP64 oracle."""
+    allf = synth.extended_chain(1000, 2000, 29)
+    w = synth.masses(2000, "uniform")
+    ref, fit = allf[:900], allf[900:]
+    with knn.KnnContext(0, prec) as ctx:
+        ctx.set_reference(ref, w)
+        d, i = ctx.query(fit, 5)
+        st = ctx.stats()
+    print(prec, "candidates per query", st["n_candidates"] / len(fit), "fallback", st["n_fallback_rows"])
+    assert st["n_fallback_rows"] < len(fit) // 2
+    d_o, i_o = oracle.knn_exact(ref, fit, w, 5)
+    check_knn(ref, fit, w, d, i, d_o, i_o)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("gen", ["metastable-2000-wide", "extended-2000", "extended-1000-wide",
                                  "metastable-1000-dups"])
 def test_window_at_large_frames(prec, gen):
