@@ -203,6 +203,32 @@ def test_symmetric_data_vectors(metric):
     assert np.abs(out[0][0] - d_o).max() < 1e-9
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_symmetric_data_vectors_along_a_line(prec):
+    """Euclidean vectors spread over 2000 units along one direction with
+    unit spacing and 1e-3 noise across it: squared norms of about 1e6
+    against neighbour distances of about 1, so approximate scores go below
+    zero (the mirrored-append and Newton-start cases of the frame tests).
+    Symmetric form = list path = oracle."""
+    rng = np.random.default_rng(31)
+    n, dim = 900, 24
+    u = rng.normal(size=dim)
+    u /= np.linalg.norm(u)
+    t = rng.permutation(np.linspace(-1000.0, 1000.0, n))
+    x = t[:, None] * u[None, :] + 1e-3 * rng.normal(size=(n, dim))
+    out = []
+    for sym in (1, 0):
+        with knn.KnnContext(0, prec) as ctx:
+            ctx.set_option(knn.OPT_SELF_SYMMETRIC, sym)
+            ctx.set_data_reference(x, "euclidean")
+            d, i = ctx.data_query(None, 6)
+            out.append((d, i, ctx.stats()))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    d_o, i_o = oracle.knn_data(x, x, 6, "euclidean")
+    assert np.abs(out[0][0] - d_o).max() < 1e-6
+
+
 def test_symmetric_then_cross_then_serving_on_one_context():
     """A self query (mirrored lists), a cross query (lists) and a served
     request share one context's workspace; each gives what it gives on a
