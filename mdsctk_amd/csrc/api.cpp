@@ -171,6 +171,7 @@ void free_frames(mdsctk_knn_ctx* c, DevFrames& f) {
   dfree(c, f.op);
   dfree(c, f.Gt_q);
   dfree(c, f.op_q);
+  dfree(c, f.rot);
   f = DevFrames();
 }
 
@@ -190,6 +191,9 @@ int alloc_frames(mdsctk_knn_ctx* c, DevFrames& f, int64_t n, int natoms, int pre
   if (prec != kExact &&
       (rc = dalloc(c, (void**)&f.op, operand_bytes(f.prec, f.n_tiles, f.n_groups), "operand")))
     return rc;
+  if (prec != kExact && c->w.metric == kMetricRMSD &&
+      (rc = dalloc(c, (void**)&f.rot, rows * 9 * sizeof(double), "principal axes")))
+    return rc;
   return 0;
 }
 
@@ -199,15 +203,19 @@ int alloc_frames(mdsctk_knn_ctx* c, DevFrames& f, int64_t n, int natoms, int pre
 bool dithered(const mdsctk_knn_ctx* c) { return c->prec == kBF16 && c->w.metric == kMetricRMSD; }
 
 // resident: the reference set (reference-side operand, plus the query-side
-// one for self queries when dithered); otherwise a query batch.
-int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st, bool resident) {
+// one for self queries when dithered); otherwise a query batch.  axes: the
+// operands hold each frame in its principal axes (fitted scores; f.rot).
+int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st, bool resident, bool axes) {
+  double* rot = axes ? f.rot : nullptr;
+  if (axes && !rot) return fail(c, MDSCTK_ERR_ARG, "prep: no principal-axes buffer");
   // The exact form needs only centroids and G (f.op is not allocated).
   if (c->prec == kExact || !dithered(c)) {
-    HIP_TRY(c, launch_prep(f, c->w, 0, nullptr, 0, st));
+    HIP_TRY(c, launch_prep(f, c->w, 0, nullptr, 0, st, nullptr, nullptr,
+                           c->prec == kExact ? nullptr : rot));
     return 0;
   }
   if (!resident) {
-    HIP_TRY(c, launch_prep(f, c->w, -1, nullptr, 0, st));
+    HIP_TRY(c, launch_prep(f, c->w, -1, nullptr, 0, st, nullptr, nullptr, rot));
     return 0;
   }
   const size_t rows = (size_t)f.n_tiles * kTile;
@@ -219,7 +227,7 @@ int prep_frames(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st, bool resident) 
   if (!f.Gt_q && (rc = dalloc(c, (void**)&f.Gt_q, rows * sizeof(double), "query-side G~")))
     return rc;
   const DevFrames q = query_view(f);
-  HIP_TRY(c, launch_prep(f, c->w, +1, &q, -1, st));
+  HIP_TRY(c, launch_prep(f, c->w, +1, &q, -1, st, nullptr, nullptr, rot));
   return 0;
 }
 
@@ -231,6 +239,65 @@ int resident_gmax(mdsctk_knn_ctx* c, DevFrames& f, hipStream_t st) {
   HIP_TRY(c, launch_max_gt(f, f.gmax, st));
   return 0;
 }
+
+// Fitted RMSD scores in a contraction form take the resident operands in
+// principal axes, and query frames are prepared the same way.
+bool axes_form(const mdsctk_knn_ctx* c) { return c->w.metric == kMetricRMSD && c->prec != kExact; }
+
+// The resident operands a query with these flags contracts against.
+const DevFrames& ref_ops(const mdsctk_knn_ctx* c, bool nofit) {
+  return (c->ref_axes && nofit) ? c->ref_plain : c->ref;
+}
+
+void free_plain(mdsctk_knn_ctx* c) {
+  DevFrames& p = c->ref_plain;
+  dfree(c, p.cen);
+  dfree(c, p.G);
+  dfree(c, p.Gt);
+  dfree(c, p.op);
+  dfree(c, p.op_q);
+  dfree(c, p.Gt_q);
+  dfree(c, p.gmax);
+  p = DevFrames();
+}
+
+}  // namespace
+
+// The --nofit operands of the resident set (frames as given), built on first
+// use: their own centroids, G, G~, operands and max G~; the coordinates are
+// the resident set's.  Synchronous, so that a batch on any stream may use
+// them once this returns.
+int ensure_plain(mdsctk_knn_ctx* c, hipStream_t st) {
+  if (!c->ref_axes) return 0;
+  std::lock_guard<std::mutex> g(c->plain_mu);
+  if (c->ref_plain.op) return 0;
+  DevFrames p = c->ref;
+  p.owns_xyz = false;
+  p.cen = p.G = p.Gt = p.Gt_q = p.gmax = p.rot = nullptr;
+  p.op = p.op_q = nullptr;
+  const size_t rows = (size_t)p.n_tiles * kTile;
+  const size_t opb = operand_bytes(p.prec, p.n_tiles, p.n_groups);
+  int rc = dalloc(c, (void**)&p.cen, rows * 3 * sizeof(double), "plain centroids");
+  if (!rc) rc = dalloc(c, (void**)&p.G, rows * sizeof(double), "plain G terms");
+  if (!rc) rc = dalloc(c, (void**)&p.Gt, rows * sizeof(double), "plain G~ terms");
+  if (!rc) rc = dalloc(c, (void**)&p.op, opb, "plain operand");
+  if (!rc && c->ref.op_q) rc = dalloc(c, (void**)&p.op_q, opb, "plain query-side operand");
+  if (!rc && c->ref.op_q) rc = dalloc(c, (void**)&p.Gt_q, rows * sizeof(double), "plain query-side G~");
+  c->ref_plain = p;
+  if (rc) {
+    free_plain(c);
+    return rc;
+  }
+  if ((rc = prep_frames(c, c->ref_plain, st, true, false)) ||
+      (rc = resident_gmax(c, c->ref_plain, st))) {
+    free_plain(c);
+    return rc;
+  }
+  HIP_TRY(c, hipStreamSynchronize(st));
+  return 0;
+}
+
+namespace {
 
 int set_weights(mdsctk_knn_ctx* c, const float* weights, int natoms) {
   std::vector<float> w(natoms), sw(natoms);
@@ -271,6 +338,8 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
   server_stop(c);
   HIP_TRY(c, hipSetDevice(c->device));
   free_frames(c, c->ref);
+  free_plain(c);
+  c->ref_axes = false;
   release_mirror_lists(c);
   dfree(c, c->data_shift);
   c->data_shift = nullptr;
@@ -284,7 +353,8 @@ int set_reference_impl(mdsctk_knn_ctx* c, const float* xyz, bool on_device, int6
                             c->main.st));
   c->ref.owns_xyz = true;
   if ((rc = alloc_frames(c, c->ref, n_ref, natoms, c->prec))) return rc;
-  if ((rc = prep_frames(c, c->ref, c->main.st, true))) return rc;
+  c->ref_axes = axes_form(c);
+  if ((rc = prep_frames(c, c->ref, c->main.st, true, c->ref_axes))) return rc;
   if ((rc = resident_gmax(c, c->ref, c->main.st))) return rc;
   HIP_TRY(c, hipStreamSynchronize(c->main.st));
   c->has_ref = true;
@@ -334,6 +404,8 @@ int set_data_reference_impl(mdsctk_knn_ctx* c, const double* x, int64_t n_ref, i
   server_stop(c);
   HIP_TRY(c, hipSetDevice(c->device));
   free_frames(c, c->ref);
+  free_plain(c);
+  c->ref_axes = false;
   release_mirror_lists(c);
   dfree(c, c->data_shift);
   c->data_shift = nullptr;
@@ -364,7 +436,7 @@ int set_data_reference_impl(mdsctk_knn_ctx* c, const double* x, int64_t n_ref, i
     HIP_TRY(c, launch_column_mean(f.x64, n_ref, vdim, c->data_shift, st));
   }
   HIP_TRY(c, launch_data_prep(f, metric, c->data_shift, st));
-  if ((rc = prep_frames(c, f, st, true))) return rc;
+  if ((rc = prep_frames(c, f, st, true, false))) return rc;
   if ((rc = resident_gmax(c, f, st))) return rc;
   HIP_TRY(c, hipStreamSynchronize(st));
   c->has_ref = true;
@@ -470,9 +542,12 @@ int slot_frames(mdsctk_knn_ctx* c, Slot& s, int64_t n_fit) {
   return 0;
 }
 
-int prep_slot(mdsctk_knn_ctx* c, Slot& s) { return prep_frames(c, s.fit, s.st, false); }
+int prep_slot(mdsctk_knn_ctx* c, Slot& s, bool nofit) {
+  return prep_frames(c, s.fit, s.st, false, c->ref_axes && !nofit);
+}
 
-int stage_input(mdsctk_knn_ctx* c, Slot& s, const void* in, bool on_device, int64_t n_fit) {
+int stage_input(mdsctk_knn_ctx* c, Slot& s, const void* in, bool on_device, int64_t n_fit,
+                bool nofit) {
   int rc = slot_frames(c, s, n_fit);
   if (rc) return rc;
   const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
@@ -484,7 +559,7 @@ int stage_input(mdsctk_knn_ctx* c, Slot& s, const void* in, bool on_device, int6
                               s.st));
     HIP_TRY(c, launch_data_prep(s.fit, c->w.metric, c->data_shift, s.st));
   }
-  return prep_slot(c, s);
+  return prep_slot(c, s, nofit);
 }
 
 int contract_grid(const mdsctk_knn_ctx* c) {
@@ -574,9 +649,10 @@ int enqueue_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, const FusedPl
                   double* d_dist, int* d_idx, bool timing) {
   const DevFrames& fit = s.fit;
   HIP_TRY(c, launch_init_lists(p.L, fit.n, s.st));
-  HIP_TRY(c, launch_eps(fit, c->ref, p.kappa, p.eps2, s.n_cand, s.st));
+  const DevFrames& R = ref_ops(c, q.nofit);
+  HIP_TRY(c, launch_eps(fit, R, p.kappa, p.eps2, s.n_cand, s.st));
   if (timing) HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
-  HIP_TRY(c, launch_contract(c->ref, fit, p.L, p.eps2, q.k1, p.phases, q.nofit, nullptr, p.grid,
+  HIP_TRY(c, launch_contract(R, fit, p.L, p.eps2, q.k1, p.phases, q.nofit, nullptr, p.grid,
                              s.st));
   if (timing) HIP_TRY(c, hipEventRecord(s.ev[3], s.st));
   HIP_TRY(c, launch_merge(c->ref, fit, c->w, p.L, p.eps2, q.k1, q.k_out, q.rank_off, q.nofit,
@@ -696,13 +772,14 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
                                                    : kappa_data(c->prec, c->ref.natoms)) *
                        c->eps_scale;
   const int grid = contract_grid(c);
-  HIP_TRY(c, launch_eps_sym(fit, c->ref, kappa, gmax_q, eps2, s.n_cand, s.st));
+  const DevFrames& R = ref_ops(c, q.nofit);
+  HIP_TRY(c, launch_eps_sym(fit, R, kappa, gmax_q, eps2, s.n_cand, s.st));
   HIP_TRY(c, hipEventRecord(s.ev[2], s.st));
   // landmark pass into the first 4 lists' worth of the list workspace
   Lists LL = L;
   LL.nl = 4;
   HIP_TRY(c, launch_init_lists(LL, n, s.st));
-  HIP_TRY(c, launch_contract(c->ref, fit, LL, eps2, q.k1, lstride, q.nofit, nullptr, grid, s.st,
+  HIP_TRY(c, launch_contract(R, fit, LL, eps2, q.k1, lstride, q.nofit, nullptr, grid, s.st,
                              nullptr, 1));
   HIP_TRY(c, launch_landmark_cap(LL, n, q.k1, eps2, lstride, L.thr, label, s.st));
   // rows grouped by nearest landmark: the merge takes them in this order (it
@@ -711,7 +788,7 @@ int launch_fused_sym(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, double* d_
   const bool ordered = n >= 512;
   if (ordered) HIP_TRY(c, launch_row_order(label, n, nlab, counts, order, s.st));
   // the main pass's frames, windows and thresholds
-  DevFrames mref = c->ref, mfit = fit;
+  DevFrames mref = R, mfit = fit;
   Lists LM = L;
   float* meps2 = eps2;
   if (canon) {
@@ -787,7 +864,7 @@ int relist_rows(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, std::vector<int
   HIP_TRY(c, hipMemcpyAsync(d_rows, rows.data(), nr * sizeof(int), hipMemcpyHostToDevice, s.st));
   if ((rc = slot_frames(c, s, nr))) return rc;
   HIP_TRY(c, launch_gather_frames(c->ref.xyz, d_rows, nr, c->ref.natoms, s.fit.xyz, s.st));
-  if ((rc = prep_slot(c, s))) return rc;
+  if ((rc = prep_slot(c, s, q.nofit))) return rc;
   FusedInfo info;
   if ((rc = launch_fused(c, s, q, t_dist, t_idx, &info))) return rc;
   HIP_TRY(c, launch_scatter_rows(t_dist, t_idx, d_rows, nr, q.k_out, d_dist, d_idx, s.st));
@@ -889,6 +966,18 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
   // A query that is not a self query does not need the self query's
   // mirrored lists; their memory goes back to the lists it does need.
   if (xyz && !cv) release_mirror_lists(c);
+  // --nofit through the contraction: the resident operands as given (built
+  // once).  Without allocation (cv) and without them, the query takes the
+  // exact rows instead.
+  bool plain_missing = false;
+  if (q.nofit && c->ref_axes) {
+    std::lock_guard<std::mutex> g(c->plain_mu);
+    plain_missing = !c->ref_plain.op;
+  }
+  if (plain_missing && q.fused && !q.full && !cv) {
+    if ((rc = ensure_plain(c, c->main.st))) return rc;
+    plain_missing = false;
+  }
   Slot& s = c->main;
   mdsctk_knn_stats S = {};
   S.n_pairs = n_fit * c->ref.n;
@@ -920,16 +1009,19 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
       f.G = (double*)cv->take(rows * sizeof(double));
       f.Gt = (double*)cv->take(rows * sizeof(double));
       if (c->prec != kExact) f.op = (uint4*)cv->take(operand_bytes(f.prec, f.n_tiles, f.n_groups));
-      if (!cs.n_cand || !f.cen || !f.G || !f.Gt || (c->prec != kExact && !f.op))
+      const bool axes = c->ref_axes && !q.nofit;
+      if (axes) f.rot = (double*)cv->take(rows * 9 * sizeof(double));
+      if (!cs.n_cand || !f.cen || !f.G || !f.Gt || (c->prec != kExact && !f.op) ||
+          (axes && !f.rot))
         return fail(c, MDSCTK_ERR_ARG,
                     "query (fixed workspace): %lld query frames of %d atoms do not fit the %zu-byte "
                     "fixed region",
                     (long long)n_fit, f.natoms, cv->size);
-      if ((rc = prep_frames(c, f, cs.st, false))) return rc;
+      if ((rc = prep_frames(c, f, cs.st, false, axes))) return rc;
       fitp = &cs.fit;
     }
   } else if (xyz) {
-    if ((rc = stage_input(c, s, xyz, on_device, n_fit))) return rc;
+    if ((rc = stage_input(c, s, xyz, on_device, n_fit, q.nofit))) return rc;
     fitp = &s.fit;
   }
   Slot& ws = cv ? cs : s;
@@ -953,9 +1045,9 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
   // self query takes the candidate lists.
   int fused_rc = 1;  // 1: the fused pipeline did not run
   FusedInfo info;
-  if (!q.full && q.fused) {
+  if (!q.full && q.fused && !plain_missing) {
     DevFrames saved = ws.fit;
-    if (!xyz) ws.fit = query_view(c->ref);
+    if (!xyz) ws.fit = query_view(ref_ops(c, q.nofit));
     if (!xyz && c->self_sym && !cv) fused_rc = launch_fused_sym(c, ws, q, d_dist, d_idx, &info);
     if (fused_rc == 1) fused_rc = launch_fused(c, ws, q, d_dist, d_idx, &info, cv);
     if (!xyz) ws.fit = saved;
@@ -1069,11 +1161,13 @@ int debug_scores_impl(mdsctk_knn_ctx* c, const void* xyz_nm, bool data, int64_t 
   HIP_TRY(c, hipSetDevice(c->device));
   Slot& s = c->main;
   const bool nofit = flags & MDSCTK_KNN_NOFIT;
-  const DevFrames self = query_view(c->ref);
-  const DevFrames* fit = &self;
   int rc;
+  if (nofit && (rc = ensure_plain(c, s.st))) return rc;
+  const DevFrames& R = ref_ops(c, nofit);
+  const DevFrames self = query_view(R);
+  const DevFrames* fit = &self;
   if (xyz_nm) {
-    if ((rc = stage_input(c, s, xyz_nm, false, n_fit))) return rc;
+    if ((rc = stage_input(c, s, xyz_nm, false, n_fit, nofit))) return rc;
     fit = &s.fit;
   }
   const size_t n = (size_t)n_fit * c->ref.n;
@@ -1082,7 +1176,7 @@ int debug_scores_impl(mdsctk_knn_ctx* c, const void* xyz_nm, bool data, int64_t 
   Lists L;
   L.nl = 4;
   L.cap = 256;
-  hipError_t e = launch_contract(c->ref, *fit, L, nullptr, 1, 1, nofit, (double*)buf,
+  hipError_t e = launch_contract(R, *fit, L, nullptr, 1, 1, nofit, (double*)buf,
                                  std::max(8, c->n_cu & ~7), s.st);  // any grid is valid
   if (e == hipSuccess)
     e = hipMemcpyAsync(approx_s, buf, n * sizeof(double), hipMemcpyDeviceToHost, s.st);

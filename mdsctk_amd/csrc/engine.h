@@ -124,7 +124,16 @@ struct mdsctk_knn_ctx {
   int mirror_cap = mdsctk::kMirrorCap;  // entries of each frame's mirrored list
 
   mdsctk::DevWeights w;  // masses, or unit masses plus the metric for data vectors
+  // The resident set.  For fitted RMSD scores in a contraction form its
+  // operands hold every frame in its own principal axes (ref_axes), as do
+  // the query frames of fitted queries: the scores do not change, and the
+  // contraction's lambda_max bound becomes nearly exact.  --nofit scores
+  // need the frames as given: ref_plain holds those operands (and its own
+  // centroids, G and G~), built when a --nofit query first needs them.
   mdsctk::DevFrames ref;
+  mdsctk::DevFrames ref_plain;
+  bool ref_axes = false;
+  std::mutex plain_mu;
   double* data_shift = nullptr;  // data vectors, Euclidean: the reference set's mean vector
   bool has_ref = false;
   mdsctk::Slot main;  // workspace of the synchronous entry points
@@ -167,8 +176,12 @@ int resolve_shape(mdsctk_knn_ctx* c, int32_t k, uint32_t flags, QueryShape* q);
 // Make the slot's query frame set hold n_fit frames of the resident
 // set's atom count (grow-only), with its tile count trimmed to n_fit.
 int slot_frames(mdsctk_knn_ctx* c, Slot& s, int64_t n_fit);
-// Centring / operand pass over the slot's query frames (async).
-int prep_slot(mdsctk_knn_ctx* c, Slot& s);
+// Centring / operand pass over the slot's query frames (async); fitted
+// queries (nofit false) on a resident set in principal axes take the
+// frames in theirs.
+int prep_slot(mdsctk_knn_ctx* c, Slot& s, bool nofit);
+// The resident set's --nofit operands, built on first use (synchronous).
+int ensure_plain(mdsctk_knn_ctx* c, hipStream_t st);
 // Contraction grid (workgroups) and reference phases for a batch of
 // `fit_tiles` query tiles; candidate-list entries a batch of n_fit frames
 // needs (each entry 8 bytes; the headers take 8 bytes per list).

@@ -80,6 +80,9 @@ struct DevFrames {
   // query-side scaling, which a self query uses as its query operand
   uint4* op_q = nullptr;
   double* Gt_q = nullptr;
+  // frames of fitted RMSD scores: each frame's principal axes, [rows][9],
+  // written by the prep pass when it builds operands in those axes
+  double* rot = nullptr;
 };
 
 // The resident set seen as a query set (self query): the query-side operand

@@ -341,9 +341,11 @@ __device__ void canonical_axes(const double* t, double* q) {
   // their axes such frames align along one axis, and the contraction's
   // approximate scores of aligned, nearly collinear frames of 1000 atoms and
   // more were measured at up to 66 times the window (as given: under half
-  // of it; every other generator in its axes: under 0.6).  Any proper
-  // rotation per frame leaves the scores unchanged.
-  if (!(a[o[1]][o[1]] >= 1e-4 * a[o[0]][o[0]])) {
+  // of it; every other generator in its axes: under 0.6).  A plane to within
+  // the same ratio (third moment) keeps it too, as the same kind of
+  // degenerate frame.  Any proper rotation per frame leaves the scores
+  // unchanged.
+  if (!(a[o[1]][o[1]] >= 1e-4 * a[o[0]][o[0]] && a[o[2]][o[2]] >= 1e-4 * a[o[0]][o[0]])) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) q[i] = (i % 4 == 0) ? 1.0 : 0.0;
     return;

@@ -148,6 +148,7 @@ class HipServeDevice {
       if (e != hipSuccess) return hip_fail(c_, e, "serve: query upload");
     }
     if (q.fused) {
+      if (q.nofit && (rc = ensure_plain(c_, s.st))) return rc;
       FusedPlan plan;
       if ((rc = plan_fused(c_, s, q, &plan))) return rc;
       if (c_->serve_share_cus && slots_.size() > 1)
@@ -155,7 +156,7 @@ class HipServeDevice {
       if (c_->serve_graphs) {
         if ((rc = replay_graph(sl, q, plan))) return rc;
       } else {
-        if ((rc = prep_slot(c_, s))) return rc;
+        if ((rc = prep_slot(c_, s, q.nofit))) return rc;
         if ((rc = enqueue_fused(c_, s, q, plan, sl.d_dist, sl.d_idx, false))) return rc;
       }
       sl.d_fb = plan.d_fallback;
@@ -163,7 +164,7 @@ class HipServeDevice {
           hipSuccess)
         return fail(c_, MDSCTK_ERR_HIP, "serve: flag copy");
     } else {
-      if ((rc = prep_slot(c_, s))) return rc;
+      if ((rc = prep_slot(c_, s, q.nofit))) return rc;
       if ((rc = exact_rows_select(c_, s, s.fit, nullptr, (int)b.n, q, sl.d_dist, sl.d_idx)))
         return rc;
       std::memset(sl.h_fb, 0, b.n * sizeof(int));
@@ -241,8 +242,9 @@ class HipServeDevice {
   size_t slot_bytes(int64_t limit, const QueryShape& q) const {
     const double natoms = c_->ref.natoms;
     const double frames = (double)((limit + 127) / 128 * 128);
-    // xyz + operand (12 B per atom each) + per-frame terms + outputs
-    double b = frames * (24.0 * natoms + 64 + 12.0 * q.k_out);
+    // xyz + operand (12 B per atom each) + per-frame terms (incl. the 72-byte
+    // principal axes) + outputs
+    double b = frames * (24.0 * natoms + 136 + 12.0 * q.k_out);
     if (q.fused) {
       size_t ent, hdr;
       list_need(limit, q, &ent, &hdr);
@@ -323,7 +325,7 @@ class HipServeDevice {
     Slot& s = sl.s;
     hipError_t e = hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(c_, e, "serve: graph capture");
-    int rc = prep_slot(c_, s);
+    int rc = prep_slot(c_, s, q.nofit);
     if (!rc) rc = enqueue_fused(c_, s, q, plan, sl.d_dist, sl.d_idx, false);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s.st, &g);
