@@ -975,8 +975,14 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
     plain_missing = !c->ref_plain.op;
   }
   if (plain_missing && q.fused && !q.full && !cv) {
-    if ((rc = ensure_plain(c, c->main.st))) return rc;
-    plain_missing = false;
+    rc = ensure_plain(c, c->main.st);
+    if (rc == 0) {
+      plain_missing = false;
+    } else if (rc == MDSCTK_ERR_NOMEM) {
+      tl_ctx_err.erase(c);  // no room for them: the exact rows instead
+    } else {
+      return rc;
+    }
   }
   Slot& s = c->main;
   mdsctk_knn_stats S = {};

@@ -147,8 +147,14 @@ class HipServeDevice {
                                     hipMemcpyHostToDevice, s.st);
       if (e != hipSuccess) return hip_fail(c_, e, "serve: query upload");
     }
-    if (q.fused) {
-      if (q.nofit && (rc = ensure_plain(c_, s.st))) return rc;
+    // --nofit through the contraction needs the resident set's operands as
+    // given; without room for them the batch takes the exact rows
+    bool fused = q.fused;
+    if (fused && q.nofit && (rc = ensure_plain(c_, s.st))) {
+      if (rc != MDSCTK_ERR_NOMEM) return rc;
+      fused = false;
+    }
+    if (fused) {
       FusedPlan plan;
       if ((rc = plan_fused(c_, s, q, &plan))) return rc;
       if (c_->serve_share_cus && slots_.size() > 1)
