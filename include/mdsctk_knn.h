@@ -127,6 +127,8 @@ typedef struct mdsctk_knn_stats {
   int64_t n_mirrored_max;   /* most appended to one frame's mirrored list */
   int64_t n_relisted_rows;  /* rows recomputed through the candidate lists (a mirrored
                                list overflowed) */
+  int64_t device_bytes;     /* device memory the context holds when the stats are read
+                               (the amount MDSCTK_OPT_HBM_BUDGET caps) */
 } mdsctk_knn_stats;
 
 /* Create a context on HIP device `device` with the given MDSCTK_PREC_*.

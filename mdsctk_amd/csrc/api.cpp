@@ -886,7 +886,22 @@ int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const in
   const int64_t n_ref = c->ref.n;
   const size_t row_bytes = (size_t)n_ref * sizeof(double);
   int batch = (int)std::max<int64_t>(1, std::min<int64_t>(nrows, (int64_t)(1ull << 30) / row_bytes));
+  // k+1 beyond the LDS select: compaction and a segmented sort in device
+  // memory (the reference has no bound on k, knn_rms.cpp:131, 242)
+  const bool sorted_path = q.k1 > kSelectLdsMaxK1;
   void* ws = nullptr;
+  std::unique_lock<std::mutex> lk(c->mem_mu);
+  if (!cv && c->budget) {
+    // Under an HBM budget the rows go through in batches that fit what the
+    // rest of the context leaves of it (this scratch is regrown, so its
+    // current size counts as free); the sort's own scratch is estimated
+    const size_t mine = s.ws_rows_bytes + (sorted_path ? s.ws_sort_bytes : 0);
+    const size_t other = c->held - std::min(c->held, mine);
+    const size_t room = c->budget > other ? c->budget - other : 0;
+    const size_t per_row = row_bytes + sizeof(int) + (sorted_path ? 24 * (size_t)q.k1 + 4 : 0);
+    batch = (int)std::max<int64_t>(1, std::min<int64_t>(batch, (int64_t)(room / per_row)));
+  }
+  lk.unlock();
   if (cv) {
     const size_t room = cv->left() > 256 ? cv->left() - 256 : 0;
     batch = (int)std::min<int64_t>(batch, (int64_t)(room / (row_bytes + sizeof(int))));
@@ -904,9 +919,6 @@ int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const in
   }
   double* srow = (double*)ws;
   int* drows = (int*)((char*)ws + row_bytes * batch);
-  // k+1 beyond the LDS select: compaction and a segmented sort in device
-  // memory (the reference has no bound on k, knn_rms.cpp:131, 242)
-  const bool sorted_path = q.k1 > kSelectLdsMaxK1;
   size_t sort_bytes = 0;
   if (sorted_path) {
     if (cv)
@@ -1259,7 +1271,10 @@ int mdsctk_knn_set_option(mdsctk_knn_ctx* c, int option, double value) {
   switch (option) {
     case MDSCTK_OPT_HBM_BUDGET:
       if (value < 0) return fail(c, MDSCTK_ERR_ARG, "budget must be >= 0");
-      c->budget = (size_t)value;
+      {
+        std::lock_guard<std::mutex> lk(c->mem_mu);  // the serving thread allocates too
+        c->budget = (size_t)value;
+      }
       return 0;
     case MDSCTK_OPT_EPS_SCALE:
       if (!(value > 0)) return fail(c, MDSCTK_ERR_ARG, "eps scale must be > 0");
@@ -1371,6 +1386,8 @@ int mdsctk_data_debug_scores(mdsctk_knn_ctx* c, const double* x, int64_t n_fit,
 int mdsctk_knn_get_stats(const mdsctk_knn_ctx* c, mdsctk_knn_stats* out) {
   if (!c || !out) return MDSCTK_ERR_ARG;
   *out = c->stats;
+  std::lock_guard<std::mutex> lk(const_cast<std::mutex&>(c->mem_mu));
+  out->device_bytes = (int64_t)c->held;
   return 0;
 }
 

@@ -40,6 +40,7 @@ struct HipSlot {
   double* h_dist = nullptr;
   int* h_idx = nullptr;
   int* h_fb = nullptr;
+  bool all_exact = false;  // the batch took the exact rows, not the contraction
   size_t h_elems = 0, h_rows = 0;
   hipEvent_t done = nullptr;
   hipGraphExec_t exec = nullptr;  // the slot's batch kernels, updated per batch
@@ -154,6 +155,7 @@ class HipServeDevice {
       if (rc != MDSCTK_ERR_NOMEM) return rc;
       fused = false;
     }
+    sl.all_exact = !fused;
     if (fused) {
       FusedPlan plan;
       if ((rc = plan_fused(c_, s, q, &plan))) return rc;
@@ -224,7 +226,8 @@ class HipServeDevice {
       const size_t cnt = (size_t)ch.nrows * q.k_out;
       std::memcpy(ch.r->out_dist + off_r, sl.h_dist + off_b, cnt * sizeof(double));
       std::memcpy(ch.r->out_idx + off_r, sl.h_idx + off_b, cnt * sizeof(int));
-      for (int64_t i = 0; i < ch.nrows; ++i) (*fb_per_chunk)[j] += sl.h_fb[ch.brow0 + i] ? 1 : 0;
+      for (int64_t i = 0; i < ch.nrows; ++i)
+        (*fb_per_chunk)[j] += (sl.all_exact || sl.h_fb[ch.brow0 + i]) ? 1 : 0;
     }
     return 0;
   }

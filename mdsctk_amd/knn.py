@@ -67,6 +67,7 @@ class Stats(ctypes.Structure):
         ("n_mirrored", ctypes.c_int64),
         ("n_mirrored_max", ctypes.c_int64),
         ("n_relisted_rows", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
     ]
 
     def as_dict(self):

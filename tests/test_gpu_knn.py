@@ -172,6 +172,57 @@ def test_full_rows_and_nofit(prec):
     check_knn(ref, fit, w, dn, iN, d_o, i_o, nofit=True)
 
 
+@pytest.mark.parametrize("served", [False, True])
+def test_nofit_without_room_for_unrotated_operands(served):
+    """A resident set held in its principal axes needs a second, unrotated
+    copy of its operands for --nofit scores.  A budget with no room for that
+    copy sends a --nofit query to the exact FP64 rows (in batches that fit the
+    budget) instead of failing: the outputs equal those of the contraction
+    with the copy, bit for bit; lifting the budget lets a later --nofit query
+    build the copy and take the contraction again.  Both contexts first run a
+    fitted query synchronously and served (one slot of up to 512 frames), so
+    that the budget is set with every workspace but the copy already held."""
+    ref, fit, w = _case(20_000, 300, 200, 17, "backbone")
+    k, nf = 7, knn.KNN_NOFIT
+
+    def _warm(ctx):
+        ctx.set_option(knn.OPT_SERVE_SLOTS, 1)
+        ctx.set_option(knn.OPT_SERVE_MAX_FRAMES, 512)
+        ctx.set_reference(ref, w)
+        ctx.query(fit, k)
+        ctx.wait(ctx.submit(fit, k))
+        return ctx.stats()["device_bytes"]
+
+    with knn.KnnContext(0, "bf16") as ctx:
+        held_fitted = _warm(ctx)
+        d0, i0 = ctx.query(fit, k, nf)
+        st0 = ctx.stats()
+    copy = st0["device_bytes"] - held_fitted
+    assert st0["n_fallback_rows"] == 0
+    assert copy > 16 * 2**20, copy
+    with knn.KnnContext(0, "bf16") as ctx:
+        held = _warm(ctx)
+        budget = held + copy // 2
+        ctx.set_option(knn.OPT_HBM_BUDGET, budget)
+        if served:
+            r = ctx.wait(ctx.submit(fit, k, nf))
+            d1, i1, fb = r.dist, r.idx, r.info["n_fallback_rows"]
+        else:
+            d1, i1 = ctx.query(fit, k, nf)
+            fb = ctx.stats()["n_fallback_rows"]
+        assert ctx.stats()["device_bytes"] <= budget
+        assert fb == fit.shape[0]
+        ctx.set_option(knn.OPT_HBM_BUDGET, 0)
+        d2, i2 = ctx.query(fit, k, nf)
+        assert ctx.stats()["n_fallback_rows"] == 0
+    for d, i in ((d1, i1), (d2, i2)):
+        np.testing.assert_array_equal(d, d0)
+        np.testing.assert_array_equal(i, i0)
+    rows = [0, 150, 299]
+    d_o, i_o = oracle.knn_exact(ref, fit[rows], w, k, nofit=True)
+    check_knn(ref, fit[rows], w, d0[rows], i0[rows], d_o, i_o, nofit=True)
+
+
 def test_keep_rank0_and_k_clamp():
     ref, fit, w = _case(20, 9, 16, 9)
     with knn.KnnContext(0, "bf16") as ctx:
