@@ -6,7 +6,8 @@ query order):
                    near-duplicates at 1e-4 nm, uniform masses), k = 64,
                    split-BF16 self query: a 4096-frame set through the
                    symmetric form and through the candidate lists, both
-                   against the FP64 oracle; and the full 200,000-frame set
+                   against the FP64 oracle (also split-FP16 and plain
+                   FP32); and the full 200,000-frame set
                    with size-independent checks plus sampled oracle rows.
   serving form     a resident 100,000 x 500-atom set, requests of 1, 1000
                    and 4096 frames together and each alone (bit-identical),
@@ -32,8 +33,8 @@ pytestmark = pytest.mark.gpu
 TOL_NM = 1e-5
 
 
-def _self_query(frames, w, k, sym):
-    with knn.KnnContext(0, "bf16") as ctx:
+def _self_query(frames, w, k, sym, prec="bf16"):
+    with knn.KnnContext(0, prec) as ctx:
         ctx.set_option(knn.OPT_SELF_SYMMETRIC, 1 if sym else 0)
         ctx.set_reference(frames, w)
         d, i = ctx.query(None, k)
@@ -49,20 +50,35 @@ def _row_properties(d, i, n):
     assert (np.diff(s, axis=1) > 0).all(), "an index repeats within a row"
 
 
-def test_precision_form_4096_frames_symmetric_and_lists_vs_oracle():
-    n, natoms, k = 4096, 1000, 64
-    frames = synth.metastable_trajectory(n, natoms, 2026)
-    w = synth.masses(natoms, "uniform")
-    d_s, i_s, st_s = _self_query(frames, w, k, True)
-    d_l, i_l, st_l = _self_query(frames, w, k, False)
-    print("symmetric: pairs", st_s["n_pairs_computed"], "candidates", st_s["n_candidates"],
+_C3_SMALL = {}
+
+
+def _c3_small():
+    """The 4096-frame metastable set and its FP64 oracle rows (k = 64),
+    computed once for the module."""
+    if not _C3_SMALL:
+        n, natoms, k = 4096, 1000, 64
+        frames = synth.metastable_trajectory(n, natoms, 2026)
+        w = synth.masses(natoms, "uniform")
+        _C3_SMALL.update(frames=frames, w=w, k=k, oracle=oracle.knn_exact(frames, frames, w, k))
+    return _C3_SMALL
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32"])
+def test_precision_form_4096_frames_symmetric_and_lists_vs_oracle(prec):
+    c = _c3_small()
+    frames, w, k = c["frames"], c["w"], c["k"]
+    n = frames.shape[0]
+    d_s, i_s, st_s = _self_query(frames, w, k, True, prec)
+    d_l, i_l, st_l = _self_query(frames, w, k, False, prec)
+    print(prec, "symmetric: pairs", st_s["n_pairs_computed"], "candidates", st_s["n_candidates"],
           "fallback rows", st_s["n_fallback_rows"], "| lists: candidates", st_l["n_candidates"],
           "fallback rows", st_l["n_fallback_rows"])
     assert st_s["n_pairs_computed"] < st_s["n_pairs"]
     assert st_l["n_pairs_computed"] == n * n
     np.testing.assert_array_equal(i_s, i_l)
     np.testing.assert_array_equal(d_s, d_l)
-    d_o, i_o = oracle.knn_exact(frames, frames, w, k)
+    d_o, i_o = c["oracle"]
     err, same = check_knn(frames, frames, w, d_s, i_s, d_o, i_o)
     print(f"max |d - oracle| {err:.3e} A, same-index fraction {same:.5f}")
     _row_properties(d_s, i_s, n)
