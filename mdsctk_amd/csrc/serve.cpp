@@ -154,6 +154,7 @@ class HipServeDevice {
     if (fused && q.nofit && (rc = ensure_plain(c_, s.st))) {
       if (rc != MDSCTK_ERR_NOMEM) return rc;
       fused = false;
+      err_.clear();  // not a failure of the batch
     }
     sl.all_exact = !fused;
     if (fused) {

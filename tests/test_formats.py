@@ -12,6 +12,8 @@ GOLD = os.path.join(HERE, "golden")
 TRP_XTC = os.path.join(GOLD, "trp-cage.xtc")
 TRP_OOS = os.path.join(GOLD, "trp-cage-outofsample.xtc")
 TRP_PDB = os.path.join(GOLD, "trp-cage.pdb")
+# the reference's backbone GRO of the same system (examples/mld/traj/proteins)
+TRP_GRO = os.path.join(GOLD, "trp-cage.bb.gro")
 
 
 def xtc_info(path):
@@ -148,3 +150,16 @@ def test_topology_masses(tmp_path):
     assert rc == 0 and np.allclose(m, [14.0067, 12.011, 1.008, 22.98977])
     rc, _ = topology(str(tmp_path / "x.tpr"))
     assert rc == knn.ERR_FORMAT
+
+
+def test_reference_gro_and_pdb_give_the_same_masses():
+    """The reference's example PDB (examples/trp-cage.pdb) and backbone GRO
+    (examples/mld/traj/proteins/trp-cage.bb.gro) list the same 60 atoms in
+    the same order: the reader gives both the same masses, one per atom of
+    the trajectory."""
+    rc_p, m_p = topology(TRP_PDB)
+    rc_g, m_g = topology(TRP_GRO)
+    assert rc_p == 0 and rc_g == 0
+    np.testing.assert_array_equal(m_g, m_p)
+    rc, n, na, pr = xtc_info(TRP_XTC)
+    assert rc == 0 and na == m_g.shape[0]

@@ -12,7 +12,7 @@ import oracle
 import oracle.graph as og
 from mdsctk_amd import build
 from tests.parity import check_knn
-from tests.test_formats import TRP_OOS, TRP_PDB, TRP_XTC, topology, xtc_read
+from tests.test_formats import TRP_GRO, TRP_OOS, TRP_PDB, TRP_XTC, topology, xtc_read
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +61,18 @@ def test_knn_rms_self_query_forms_agree(tmp_path, trp):
         assert p.returncode == 0, p.stdout + p.stderr
         outs.append(((tmp_path / f"d{tag}.dat").read_bytes(), (tmp_path / f"i{tag}.dat").read_bytes()))
     assert outs[0] == outs[1] == outs[2]
+
+
+def test_knn_rms_gro_topology_writes_the_same_files(tmp_path):
+    """-p naming the reference's backbone GRO of the system writes the same
+    files as -p naming its PDB (the same atoms and masses)."""
+    outs = []
+    for tag, top in (("p", TRP_PDB), ("g", TRP_GRO)):
+        p = run(build.CLI_PATH, ["-k", "10", "-p", top, "-r", TRP_XTC, "-d", f"d{tag}.dat",
+                                 "-i", f"i{tag}.dat"], tmp_path)
+        assert p.returncode == 0, p.stdout + p.stderr
+        outs.append(((tmp_path / f"d{tag}.dat").read_bytes(), (tmp_path / f"i{tag}.dat").read_bytes()))
+    assert outs[0] == outs[1]
 
 
 def test_knn_rms_out_of_sample_and_nofit(tmp_path, trp):
