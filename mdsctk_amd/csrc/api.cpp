@@ -1386,7 +1386,7 @@ int mdsctk_data_debug_scores(mdsctk_knn_ctx* c, const double* x, int64_t n_fit,
 int mdsctk_knn_get_stats(const mdsctk_knn_ctx* c, mdsctk_knn_stats* out) {
   if (!c || !out) return MDSCTK_ERR_ARG;
   *out = c->stats;
-  std::lock_guard<std::mutex> lk(const_cast<std::mutex&>(c->mem_mu));
+  std::lock_guard<std::mutex> lk(c->mem_mu);
   out->device_bytes = (int64_t)c->held;
   return 0;
 }

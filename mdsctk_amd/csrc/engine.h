@@ -110,7 +110,7 @@ struct mdsctk_knn_ctx {
   int prec = mdsctk::kFP32;
   int n_cu = 256;
   // Device memory accounting; the serving thread allocates too.
-  std::mutex mem_mu;
+  mutable std::mutex mem_mu;
   size_t budget = 0;  // 0 = unlimited
   size_t held = 0;
   std::unordered_map<void*, size_t> allocs;
