@@ -900,6 +900,11 @@ int exact_rows_select(mdsctk_knn_ctx* c, Slot& s, const DevFrames& fit, const in
     const size_t room = c->budget > other ? c->budget - other : 0;
     const size_t per_row = row_bytes + sizeof(int) + (sorted_path ? 24 * (size_t)q.k1 + 4 : 0);
     batch = (int)std::max<int64_t>(1, std::min<int64_t>(batch, (int64_t)(room / per_row)));
+    // the sort's scratch also has a part the estimate leaves out (the radix
+    // sort's own storage): shrink until rows and scratch fit together
+    while (sorted_path && batch > 1 &&
+           (row_bytes + sizeof(int)) * batch + sort_rows_scratch(batch, q.k1) > room)
+      batch = std::max(1, batch * 3 / 4);
   }
   lk.unlock();
   if (cv) {

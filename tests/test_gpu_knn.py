@@ -172,6 +172,33 @@ def test_full_rows_and_nofit(prec):
     check_knn(ref, fit, w, dn, iN, d_o, i_o, nofit=True)
 
 
+def test_k_beyond_2048_within_a_budget():
+    """Exact rows under an HBM budget go through in batches that fit what it
+    leaves, the segmented sort's scratch included: a budget about 20 MB
+    below what one batch of all 300 rows holds (48 KB of scores and about
+    72 KB of sort keys per row for k = 3000 over 6000 frames) gives the same
+    rows, and the context stays within it."""
+    ref, fit, w = _case(6000, 300, 24, 19, "backbone")
+    k = 3000
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_reference(ref, w)
+        d0, i0 = ctx.query(fit, k)
+        full = ctx.stats()["device_bytes"]
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_reference(ref, w)
+        held = ctx.stats()["device_bytes"]
+        budget = full - 20 * 2**20
+        assert budget > held + 16 * 2**20, (held, full)  # outputs (10.8 MB) and a few rows fit
+        ctx.set_option(knn.OPT_HBM_BUDGET, budget)
+        d1, i1 = ctx.query(fit, k)
+        assert ctx.stats()["device_bytes"] <= budget
+        assert ctx.stats()["n_fallback_rows"] == fit.shape[0]
+    np.testing.assert_array_equal(d1, d0)
+    np.testing.assert_array_equal(i1, i0)
+    d_o, i_o = oracle.knn_exact(ref, fit[:4], w, k)
+    check_knn(ref, fit[:4], w, d0[:4], i0[:4], d_o, i_o)
+
+
 @pytest.mark.parametrize("served", [False, True])
 def test_nofit_without_room_for_unrotated_operands(served):
     """A resident set held in its principal axes needs a second, unrotated
