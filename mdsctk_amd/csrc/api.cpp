@@ -617,7 +617,20 @@ int plan_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, FusedPlan* p, Ca
                   (long long)n_fit, q.k1, need(p->phases), cv->left(), cv->size);
     ent = cv->take(ent_bytes_for(p->phases));
     small_p = cv->take(small_bytes_for(p->phases));
+  } else {
+    // Under an HBM budget, fewer phases (fewer lists per query) until the
+    // lists fit what the rest of the context leaves of it (these buffers
+    // are regrown, so their current size counts as free)
+    std::lock_guard<std::mutex> lk(c->mem_mu);
+    if (c->budget) {
+      const size_t mine = s.ws_lists_bytes + s.ws_small_bytes;
+      const size_t other = c->held - std::min(c->held, mine);
+      const size_t room = c->budget > other ? c->budget - other : 0;
+      while (p->phases > 1 && ent_bytes_for(p->phases) + small_bytes_for(p->phases) > room)
+        p->phases /= 2;
+    }
   }
+  // The phase count is final here: the list layout below follows it.
   Lists& L = p->L;
   L = Lists();
   L.nl = p->phases * 4;
@@ -638,7 +651,10 @@ int plan_fused(mdsctk_knn_ctx* c, Slot& s, const QueryShape& q, FusedPlan* p, Ca
   L.ovf = (float*)sp; sp += n_hdr * sizeof(float);
   L.thr = (unsigned*)sp; sp += n_fit * sizeof(unsigned);
   p->eps2 = (float*)sp; sp += n_fit * sizeof(float);
-  p->d_fallback = (int*)sp;
+  p->d_fallback = (int*)sp; sp += n_fit * sizeof(int);
+  if ((size_t)(sp - (char*)small_p) > small_bytes_for(p->phases))
+    return fail(c, MDSCTK_ERR_ARG, "internal: list headers for %d lists per query exceed their %zu bytes",
+                L.nl, small_bytes_for(p->phases));
   p->kappa = (c->w.metric == kMetricRMSD ? kappa_for(c->prec, c->ref.natoms)
                                          : kappa_data(c->prec, c->ref.natoms)) *
              c->eps_scale;
@@ -1074,6 +1090,11 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
     if (!xyz && c->self_sym && !cv) fused_rc = launch_fused_sym(c, ws, q, d_dist, d_idx, &info);
     if (fused_rc == 1) fused_rc = launch_fused(c, ws, q, d_dist, d_idx, &info, cv);
     if (!xyz) ws.fit = saved;
+    if (fused_rc == MDSCTK_ERR_NOMEM && !cv) {
+      // even one phase's lists do not fit: the exact rows, in batches that do
+      tl_ctx_err.erase(c);
+      fused_rc = 1;
+    }
     if (fused_rc < 0) return fused_rc;
   }
   if (q.full) {

@@ -199,6 +199,41 @@ def test_k_beyond_2048_within_a_budget():
     check_knn(ref, fit[:4], w, d0[:4], i0[:4], d_o, i_o)
 
 
+def test_cross_query_lists_within_a_budget():
+    """A budget without room for a cross query's candidate lists: the query
+    takes fewer reference phases (fewer lists per query) until they fit and,
+    when even one phase's lists do not, the exact rows in batches that fit.
+    Both give the rows of the unconstrained query, bit for bit."""
+    ref, fit, w = _case(20_000, 4000, 40, 23, "backbone")
+    k = 7
+    with knn.KnnContext(0, "bf16") as ctx:
+        ctx.set_reference(ref, w)
+        d0, i0 = ctx.query(fit, k)
+        st0 = ctx.stats()
+    p0, cap = st0["ref_splits"], st0["list_capacity"]
+    lists = lambda ph: fit.shape[0] * ph * 4 * cap * 8
+    assert p0 >= 8 and st0["n_fallback_rows"] == 0, (p0, st0["n_fallback_rows"])
+    full = st0["device_bytes"]
+    # room for half the phases' lists; then for half of one phase's
+    for budget, fewer in ((full - lists(p0) // 2, True), (full - lists(p0) + lists(1) // 2, False)):
+        with knn.KnnContext(0, "bf16") as ctx:
+            ctx.set_reference(ref, w)
+            ctx.set_option(knn.OPT_HBM_BUDGET, budget)
+            d1, i1 = ctx.query(fit, k)
+            st1 = ctx.stats()
+        print("budget", budget, "phases", p0, "->", st1["ref_splits"], "fallback rows",
+              st1["n_fallback_rows"], "held", st1["device_bytes"])
+        assert st1["device_bytes"] <= budget
+        if fewer:
+            assert 1 <= st1["ref_splits"] <= p0 // 2
+        else:
+            assert st1["n_fallback_rows"] == fit.shape[0]
+        np.testing.assert_array_equal(d1, d0)
+        np.testing.assert_array_equal(i1, i0)
+    d_o, i_o = oracle.knn_exact(ref, fit[:6], w, k)
+    check_knn(ref, fit[:6], w, d0[:6], i0[:6], d_o, i_o)
+
+
 @pytest.mark.parametrize("served", [False, True])
 def test_nofit_without_room_for_unrotated_operands(served):
     """A resident set held in its principal axes needs a second, unrotated
