@@ -72,7 +72,9 @@ extern "C" {
 #define MDSCTK_KNN_KEEP_RANK0 4u  /* keep the nearest neighbour instead of dropping it */
 
 /* Options for mdsctk_knn_set_option. */
-#define MDSCTK_OPT_HBM_BUDGET 1   /* bytes of device memory the context may hold (0 = no limit) */
+#define MDSCTK_OPT_HBM_BUDGET 1   /* bytes of device memory the context may hold (0 = no limit);
+                                     a query that would exceed it takes fewer candidate lists,
+                                     or exact rows in batches that fit, with the same outputs */
 #define MDSCTK_OPT_EPS_SCALE 2    /* multiplier on the candidate window (default 1); rows whose
                                      re-scored candidates, or the keys just above their cut,
                                      deviate by more than the scaled window's bound are
