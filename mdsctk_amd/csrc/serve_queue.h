@@ -173,10 +173,17 @@ class ServeQueue {
     typename Device::SlotState* dev = nullptr;
   };
 
-  // Under mu_: pack pending requests into the slot's batch (arrival order,
-  // the oldest request's shape; a request larger than the room left is split
-  // by rows and its remainder goes first in the next batch).  Returns false
-  // when nothing is pending.
+  // Requests of at most this many rows (left) go first into a batch.
+  static constexpr int64_t kShortRows = 64;
+
+  // Under mu_: pack pending requests of the oldest request's shape into the
+  // slot's batch: first every whole pending request of at most kShortRows
+  // rows that fits, so that a short request waits for the batches already
+  // in flight and not also for the rows of larger requests queued ahead of
+  // it; then the rest in arrival order (a request larger than the room left
+  // is split by rows and its remainder goes first in the next batch).  A
+  // row's outputs do not depend on its batch.  Returns false when nothing is
+  // pending.
   bool form_batch(Slot& sl) {
     if (pending_.empty()) return false;
     ServeBatch<Shape>& b = sl.b;
@@ -185,6 +192,24 @@ class ServeQueue {
     const int64_t limit = dev_.frame_limit(b.q);
     int64_t n = 0;
     const double now = dev_.now();
+    auto place = [&](const std::shared_ptr<Request>& rp, int64_t take) {
+      Request& r = *rp;
+      b.chunks.push_back(ServeChunk<Shape>{rp, r.next_row, take, n});
+      if (r.t_admit < 0) r.t_admit = now;
+      ++r.n_batches;
+      r.next_row += take;
+      n += take;
+    };
+    for (auto it = pending_.begin(); it != pending_.end() && n < limit;) {
+      Request& r = **it;
+      const int64_t left = r.n - r.next_row;
+      if (Device::same_shape(r.q, b.q) && left <= kShortRows && left <= limit - n) {
+        place(*it, left);
+        it = pending_.erase(it);
+      } else {
+        ++it;
+      }
+    }
     for (auto it = pending_.begin(); it != pending_.end() && n < limit;) {
       Request& r = **it;
       if (!Device::same_shape(r.q, b.q)) {
@@ -192,11 +217,7 @@ class ServeQueue {
         continue;
       }
       const int64_t take = std::min(r.n - r.next_row, limit - n);
-      b.chunks.push_back(ServeChunk<Shape>{*it, r.next_row, take, n});
-      if (r.t_admit < 0) r.t_admit = now;
-      ++r.n_batches;
-      r.next_row += take;
-      n += take;
+      place(*it, take);
       if (r.next_row == r.n)
         it = pending_.erase(it);
       else
