@@ -208,8 +208,9 @@ int mdsctk_knn_debug_scores(mdsctk_knn_ctx* ctx, const float* xyz_nm, int64_t n_
  * (Nystrom) query form of knn_rms.cpp:231-291 with the reference set kept
  * resident.  The first submit starts a serving thread that owns
  * MDSCTK_OPT_SERVE_SLOTS batch slots, each with its own HIP stream.
- * Pending requests are admitted in arrival order into the next free slot,
- * up to MDSCTK_OPT_SERVE_MAX_FRAMES frames per batch (clamped so that the
+ * Pending requests are admitted into the next free slot, whole requests of
+ * at most 64 frames first and then the rest in arrival order, up to
+ * MDSCTK_OPT_SERVE_MAX_FRAMES frames per batch (clamped so that the
  * slots fit the HBM budget); a request larger than that is split over
  * consecutive batches by rows.  A batch is one fused launch sequence over
  * the concatenated frames of its requests, so requests that arrive while
