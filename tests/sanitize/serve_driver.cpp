@@ -61,6 +61,15 @@ class HostDevice {
       return MDSCTK_ERR_HIP;
     }
     check(b.n > 0 && b.n <= limit, "batch size within the frame limit");
+    // Whole short requests (at most 64 rows left) are packed before the
+    // rest: a chunk that completes its request with at most 64 rows comes
+    // only from that first pass, so none follows any other chunk.
+    bool other_seen = false;
+    for (const auto& ch : b.chunks) {
+      const bool short_whole = ch.nrows <= 64 && ch.row0 + ch.nrows == ch.r->n;
+      if (short_whole) check(!other_seen, "short requests packed first");
+      else other_seen = true;
+    }
     s.countdown = 1 + (int)(b.n % 3);
     return 0;
   }
