@@ -1000,8 +1000,8 @@ int query_impl(mdsctk_knn_ctx* c, const void* xyz, bool data, bool on_device, in
   // mirrored lists; their memory goes back to the lists it does need.
   if (xyz && !cv) release_mirror_lists(c);
   // --nofit through the contraction: the resident operands as given (built
-  // once).  Without allocation (cv) and without them, the query takes the
-  // exact rows instead.
+  // once).  Without them, in the allocation-free form (cv) or when they do
+  // not fit, the query takes the exact rows instead.
   bool plain_missing = false;
   if (q.nofit && c->ref_axes) {
     std::lock_guard<std::mutex> g(c->plain_mu);
